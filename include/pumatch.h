@@ -1,0 +1,157 @@
+/*
+ * pumatch.h -- C ABI of the MI355X subgraph matcher (libpumatch.so).
+ *
+ * The reference (PUMatch) exposes no library: its boundary is the CLI
+ * `test <graph-prefix> <pattern.g>` (test.cu:15-212), the binary CSR files
+ * read by GraphPreprocessor (src/graph.h:228-254), the .g pattern files read
+ * by PatternPreprocessor (src/pattern.h:67-100) and the compile-time switch
+ * LABELED (src/config.h:14).  Each entry point below replaces one step of
+ * that driver; the comment on each names the reference code it stands in for.
+ *
+ * Conventions: every function returns PM_OK (0) or a negative PM_ERR_*;
+ * pm_last_error() returns a message for the calling thread.  Handles own
+ * their host and HBM memory and are released by the matching *_destroy.
+ * Calls on one handle are serial.  No torch types cross this boundary.
+ */
+#ifndef PUMATCH_H
+#define PUMATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_MAX_PATTERN 7 /* PAT_SIZE, src/config.h:9 */
+
+enum {
+  PM_OK = 0,
+  PM_ERR_IO = -1,       /* file missing / short read (src/graph.h:219-223 exits) */
+  PM_ERR_FORMAT = -2,   /* malformed .meta.txt / .g / CSR */
+  PM_ERR_PATTERN = -3,  /* pattern violates the matcher's preconditions */
+  PM_ERR_DEVICE = -4,   /* HIP error / no device */
+  PM_ERR_ARG = -5,      /* bad argument */
+  PM_ERR_OVERFLOW = -6, /* a device work area would overflow */
+  PM_ERR_NOMEM = -7
+};
+
+/* graph flags */
+#define PM_GRAPH_LABELS 0x1    /* also read <prefix>.label.bin */
+#define PM_GRAPH_HOST_ONLY 0x2 /* keep the CSR on the host (no HBM copy) */
+
+typedef struct pm_graph pm_graph;
+typedef struct pm_pattern pm_pattern;
+
+typedef struct {
+  int32_t waves;        /* persistent wavefronts (0 = auto: 8 per CU) */
+  int32_t chunk;        /* jobs (CSR edges) claimed per atomic (0 = auto) */
+  int32_t repeats;      /* timed launches; kernel_ms is their mean (0 = 1) */
+  int32_t reserved0;
+  uint64_t edge_lo;     /* restrict jobs to CSR edges [edge_lo, edge_hi); hi=0: all */
+  uint64_t edge_hi;
+  uint64_t* per_edge;   /* optional host u64[m]: count per job edge (before x|Aut|) */
+  void* stream;         /* hipStream_t to launch on (NULL: the library's own) */
+} pm_opts;
+
+typedef struct {
+  double kernel_ms;     /* mean kernel time (hipEvents on the launch stream) */
+  double min_ms;        /* fastest of the timed launches */
+  uint64_t count;       /* match count (x|Aut| when unlabeled, test.cu:198) */
+  uint64_t jobs;        /* jobs that passed the filter (src/job_queue.h:44-61) */
+  uint64_t alg_bytes;   /* algorithmic bytes of one launch (see DESIGN) */
+  uint64_t set_ops;     /* set operations executed by one launch */
+  uint64_t link_bytes;  /* host->HBM bytes paged during the launch */
+  int32_t waves;        /* wavefronts actually launched */
+  int32_t chunk;
+} pm_result;
+
+typedef struct {
+  uint64_t n, m;            /* vertices, directed edges */
+  uint32_t max_degree;
+  int32_t has_labels;
+  int32_t on_device;
+  int32_t reserved;
+  uint64_t device_bytes;    /* HBM held by the graph */
+} pm_graph_info;
+
+const char* pm_last_error(void);
+const char* pm_version(void);
+
+/* -- graphs --------------------------------------------------------------- */
+
+/* Load <prefix>.meta.txt/.vertex.bin/.edge.bin[/.label.bin] and copy the CSR
+ * to HBM.  Replaces GraphPreprocessor(prefix) + to_gpu()
+ * (src/graph.h:34-41, :169-184, :228-254; test.cu:19-20,47). */
+int pm_graph_open(const char* prefix, uint32_t flags, pm_graph** out);
+
+/* Build a graph from host CSR arrays (copied).  labels may be NULL.  Same
+ * layout as read_bin_file (src/graph.h:228-254). */
+int pm_graph_from_csr(uint64_t n, uint64_t m, const uint64_t* rowptr, const uint32_t* colidx,
+                      const uint32_t* labels, uint32_t flags, pm_graph** out);
+
+/* Seeded synthetic generators; the result is symmetric, deduplicated,
+ * self-loop free and sorted, the layout the reference's converter produces
+ * (graph_converter/compactify.cpp:10-51).  No reference counterpart. */
+int pm_graph_generate_rmat(int32_t scale, double edge_factor, double a, double b, double c,
+                           uint64_t seed, uint32_t flags, pm_graph** out);
+int pm_graph_generate_chunglu(uint64_t n, double gamma, double dmax, double avg, uint64_t seed,
+                              uint32_t flags, pm_graph** out);
+/* uniform labels 0..nlabels-1 (nlabels <= 32); uploads them when on device */
+int pm_graph_set_uniform_labels(pm_graph* g, uint32_t nlabels, uint64_t seed);
+
+/* Write the graph in the reference's on-disk format (the files
+ * read_bin_file reads, src/graph.h:228-254). */
+int pm_graph_write(const pm_graph* g, const char* prefix);
+
+int pm_graph_info_get(const pm_graph* g, pm_graph_info* info);
+/* Borrow the host CSR arrays (valid until pm_graph_destroy). */
+int pm_graph_host_csr(const pm_graph* g, const uint64_t** rowptr, const uint32_t** colidx,
+                      const uint32_t** labels);
+/* Copy the CSR to HBM (no-op when already there). */
+int pm_graph_to_device(pm_graph* g);
+void pm_graph_destroy(pm_graph* g);
+
+/* -- patterns ------------------------------------------------------------- */
+
+/* Parse a gSpan-like .g file and plan it.  labeled=1 uses the vertex labels
+ * of the file (LABELED=true, src/config.h:14); labeled=0 sets every label to
+ * 1 and counts embeddings with symmetry breaking.  Replaces
+ * PatternPreprocessor(path) (src/pattern.h:48-58). */
+int pm_pattern_load(const char* path, int32_t labeled, pm_pattern** out);
+/* Same from the text of a .g file. */
+int pm_pattern_parse(const char* text, int32_t labeled, pm_pattern** out);
+
+typedef struct {
+  int32_t k;
+  int32_t labeled;
+  int64_t automorphisms;  /* PatternMultiplicity, src/pattern.h:236 */
+  int32_t rc_order;       /* unlabeled job filter r > c, src/job_queue.h:44 */
+  int32_t nslots;
+  int32_t degree[PM_MAX_PATTERN];
+  int32_t label[PM_MAX_PATTERN];
+  uint32_t sym_mask[PM_MAX_PATTERN];  /* phi(u) < phi(j) for bit j */
+  uint32_t nbr_mask[PM_MAX_PATTERN];  /* earlier neighbours of u */
+} pm_pattern_info;
+
+int pm_pattern_info_get(const pm_pattern* p, pm_pattern_info* info);
+void pm_pattern_destroy(pm_pattern* p);
+
+/* -- matching ------------------------------------------------------------- */
+
+void pm_opts_default(pm_opts* o);
+
+/* Count the matches of p in g on the GPU.  Replaces the timed region of
+ * test.cu:171-200 (job generation src/job_queue.h:39-125 runs on the device
+ * inside the launch). */
+int pm_match(pm_graph* g, const pm_pattern* p, const pm_opts* opts, pm_result* res);
+
+/* Run n patterns back to back on one graph; results[i] as from pm_match. */
+int pm_match_batch(pm_graph* g, const pm_pattern* const* p, int32_t n, const pm_opts* opts,
+                   pm_result* results);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PUMATCH_H */

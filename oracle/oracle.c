@@ -1,0 +1,385 @@
+/*
+ * oracle.c -- CPU oracle for the subgraph-matching hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is built into
+ * oracle/_build/liboracle.so and oracle/_build/pm_oracle and may be loaded
+ * only by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py.  The product library (pumatch_amd/) never links or calls it.
+ *
+ * It restates the count semantics of the reference matcher (PUMatch) on the
+ * CPU, as a plain backtracking enumerator with OpenMP over jobs:
+ *
+ *   pattern plan      src/pattern.h:67-122   (.g parse, identity order, degrees)
+ *                     src/pattern.h:198-262  (automorphisms, stabiliser chain,
+ *                                             symmetry-breaking pairs)
+ *   job filter        src/job_queue.h:39-64
+ *   graph loader      src/graph.h:228-254
+ *   set semantics     src/gpu_match.cu:238-319 (compute_set: "< ub" truncation,
+ *                                             label test, membership / DIFF)
+ *                     src/gpu_match.cu:386-816 (extend: which lists are
+ *                                             intersected, injectivity passes)
+ *                     src/gpu_match.cu:1018-1041 (last level only counts)
+ *   final multiply    test.cu:195-200       (x |Aut| when unlabeled)
+ *
+ * Count definition (what every function below computes):
+ *   jobs J  = directed CSR edges (r,c) with deg(r) >= d_P(0), deg(c) >= d_P(1),
+ *             some w in N(c) with deg(w) >= d_P(2); labeled: lab(r)==l_P(0),
+ *             lab(c)==l_P(1); unlabeled with L[0][1]: r > c.
+ *   labeled   sum over J of #{phi : phi(0)=r, phi(1)=c, lab(phi(u))==l_P(u),
+ *             phi(u) in N(phi(j)) for every pattern edge j<u, u>=2}
+ *             (a label-preserving homomorphism count; no injectivity, no
+ *             symmetry breaking, no multiplicity).
+ *   unlabeled |Aut(P)| * sum over J of #{injective phi as above without labels,
+ *             with phi(u) < phi(j) for every symmetry pair (j,u)}.
+ *
+ * The reference's 4096-entry slot cap (src/gpu_match.cu:302,316) is a defect
+ * and is not reproduced.
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#define OR_MAXK 7
+
+typedef struct {
+  int k;
+  int adj[OR_MAXK][OR_MAXK]; /* edge label, 0 = no edge, symmetric */
+  int vlabel[OR_MAXK];
+} or_pattern;
+
+typedef struct {
+  int k;
+  int labeled;
+  int deg[OR_MAXK];       /* pattern degree, src/pattern.h:115-121 */
+  long long aut;          /* |Aut|, src/pattern.h:236 */
+  uint32_t A[OR_MAXK];    /* earlier neighbours of u */
+  uint32_t B[OR_MAXK];    /* phi(u) < phi(j) for each j in B[u] (unlabeled) */
+  uint32_t X[OR_MAXK];    /* earlier non-neighbours of u (injectivity) */
+  uint32_t lbit[OR_MAXK]; /* 1 << l_P(u) */
+  int rc_order;           /* unlabeled job filter r > c, src/job_queue.h:44 */
+} or_plan;
+
+/* ---------------------------------------------------------------------- */
+/* plan: automorphisms and the stabiliser chain                            */
+/* ---------------------------------------------------------------------- */
+
+/* Permutations in the reference's swap-recursion order
+ * (src/pattern.h:124-141).  The order does not change the result; it is kept
+ * so the two enumerations can be compared permutation by permutation. */
+static void or_permute(int* a, int l, int r, int (*out)[OR_MAXK], int* nout, int k) {
+  if (l == r) {
+    memcpy(out[*nout], a, sizeof(int) * k);
+    (*nout)++;
+    return;
+  }
+  for (int i = l; i <= r; i++) {
+    int t = a[l]; a[l] = a[i]; a[i] = t;
+    or_permute(a, l + 1, r, out, nout, k);
+    t = a[l]; a[l] = a[i]; a[i] = t;
+  }
+}
+
+/* src/pattern.h:207-234: pp is kept when, for every vertex i, the set of j with
+ * adj[i][j]==1 equals the image set {pp[j] : adj[pp^-1(i)][j] != 0}. */
+static int or_perm_valid(const or_pattern* p, const int* pp) {
+  int k = p->k;
+  uint32_t img[OR_MAXK];
+  for (int i = 0; i < k; i++) {
+    uint32_t s = 0;
+    for (int j = 0; j < k; j++)
+      if (p->adj[i][j] != 0) s |= 1u << pp[j];
+    img[pp[i]] = s;
+  }
+  for (int i = 0; i < k; i++) {
+    uint32_t ones = 0;
+    for (int j = 0; j < k; j++)
+      if (p->adj[i][j] == 1) ones |= 1u << j;
+    if (ones != img[i]) return 0;
+  }
+  return 1;
+}
+
+/* Returns 0 on success, -1 when the pattern breaks the matcher's
+ * preconditions (k in 3..7, vertex 1 adjacent to 0, every vertex u>=1 has an
+ * earlier neighbour: src/pattern.h:148-163 builds no slot otherwise). */
+int or_plan_pattern(const or_pattern* p, int labeled, or_plan* out) {
+  memset(out, 0, sizeof(*out));
+  int k = p->k;
+  if (k < 3 || k > OR_MAXK) return -1;
+  out->k = k;
+  out->labeled = labeled;
+  for (int i = 0; i < k; i++) {
+    int d = 0;
+    for (int j = 0; j < k; j++)
+      if (p->adj[i][j] > 0) d++;
+    out->deg[i] = d;
+    out->lbit[i] = 1u << (labeled ? p->vlabel[i] : 1);
+  }
+  for (int u = 1; u < k; u++) {
+    uint32_t a = 0, x = 0;
+    for (int j = 0; j < u; j++) {
+      if (p->adj[u][j]) a |= 1u << j; else x |= 1u << j;
+    }
+    if (a == 0) return -1;
+    out->A[u] = a;
+    out->X[u] = x;
+  }
+  if (!(out->A[1] & 1u)) return -1;
+
+  /* k! <= 5040 permutations */
+  static __thread int perms[5040][OR_MAXK];
+  int nperm = 0, base[OR_MAXK];
+  for (int i = 0; i < k; i++) base[i] = i;
+  or_permute(base, 0, k - 1, perms, &nperm, k);
+  static __thread int valid[5040];
+  int nvalid = 0;
+  for (int t = 0; t < nperm; t++)
+    if (or_perm_valid(p, perms[t])) valid[nvalid++] = t;
+  out->aut = nvalid;
+
+  /* stabiliser chain, src/pattern.h:238-252 */
+  int L[OR_MAXK][OR_MAXK];
+  memset(L, 0, sizeof(L));
+  for (int v = 0; v < k; v++) {
+    int keep = 0;
+    for (int t = 0; t < nvalid; t++) {
+      const int* x = perms[valid[t]];
+      if (x[v] == v) valid[keep++] = valid[t];
+      else L[v][x[v]] = 1;
+    }
+    nvalid = keep;
+  }
+  /* src/pattern.h:254-261: partial[level-1][0] bit j <=> L[j][level] */
+  for (int u = 1; u < k; u++) {
+    uint32_t b = 0;
+    for (int j = u - 1; j >= 0; j--)
+      if (L[j][u]) b |= 1u << j;
+    out->B[u] = b;
+  }
+  out->rc_order = (out->B[1] & 1u) ? 1 : 0;
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* enumeration                                                             */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+  uint64_t n;
+  const uint64_t* rowptr;
+  const uint32_t* colidx;
+  const uint32_t* lab; /* raw label values (0..31), NULL when unlabeled */
+} or_graph;
+
+static inline uint64_t or_deg(const or_graph* g, uint32_t v) {
+  return g->rowptr[v + 1] - g->rowptr[v];
+}
+
+/* elements of a (sorted) strictly below ub: the reference's upper_bound,
+ * src/gpu_match.cu:185-202 */
+static inline uint64_t or_below(const uint32_t* a, uint64_t n, uint32_t ub) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < ub) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+static inline int or_member(const uint32_t* a, uint64_t n, uint32_t x) {
+  uint64_t i = or_below(a, n, x);
+  return i < n && a[i] == x;
+}
+
+/* Per-thread DFS state.  P[d][u] holds the running intersection
+ * of N(phi(j)) over the earlier neighbours j<=d of future vertex u. */
+typedef struct {
+  const or_graph* g;
+  const or_plan* pl;
+  uint32_t phi[OR_MAXK];
+  const uint32_t* set[OR_MAXK][OR_MAXK]; /* [depth][u] */
+  uint64_t len[OR_MAXK][OR_MAXK];
+  uint32_t* buf[OR_MAXK][OR_MAXK];       /* owned storage, capacity maxdeg */
+} or_state;
+
+/* merge intersection of two sorted lists, elements < ub only */
+static uint64_t or_intersect(const uint32_t* a, uint64_t na, const uint32_t* b, uint64_t nb,
+                             uint32_t ub, uint32_t* out) {
+  na = or_below(a, na, ub);
+  nb = or_below(b, nb, ub);
+  uint64_t i = 0, j = 0, o = 0;
+  while (i < na && j < nb) {
+    if (a[i] < b[j]) i++;
+    else if (a[i] > b[j]) j++;
+    else { out[o++] = a[i]; i++; j++; }
+  }
+  return o;
+}
+
+/* upper bound on phi(u) from its symmetry pairs among matched vertices */
+static inline uint32_t or_bound(const or_state* s, int u, int d) {
+  uint32_t ub = 0xFFFFFFFFu;
+  if (s->pl->labeled) return ub;
+  uint32_t b = s->pl->B[u];
+  for (int j = 0; j <= d; j++)
+    if ((b >> j & 1u) && s->phi[j] < ub) ub = s->phi[j];
+  return ub;
+}
+
+/* candidate x for vertex u is admissible (label / injectivity) */
+static inline int or_admit(const or_state* s, int u, uint32_t x) {
+  if (s->pl->labeled) return (1u << s->g->lab[x]) == s->pl->lbit[u];
+  uint32_t xm = s->pl->X[u];
+  for (int j = 0; j < u; j++)
+    if ((xm >> j & 1u) && s->phi[j] == x) return 0;
+  return 1;
+}
+
+/* Build the running sets at depth d (vertices 0..d matched). */
+static void or_extend(or_state* s, int d) {
+  const or_plan* pl = s->pl;
+  const or_graph* g = s->g;
+  uint32_t v = s->phi[d];
+  const uint32_t* nv = g->colidx + g->rowptr[v];
+  uint64_t dv = or_deg(g, v);
+  for (int u = d + 1; u < pl->k; u++) {
+    uint32_t a = pl->A[u] & ((2u << d) - 1u);
+    if (!(a >> d & 1u)) { /* unchanged from depth d-1 */
+      s->set[d][u] = d > 0 ? s->set[d - 1][u] : NULL;
+      s->len[d][u] = d > 0 ? s->len[d - 1][u] : 0;
+      continue;
+    }
+    uint32_t ub = or_bound(s, u, d);
+    if (a == (1u << d)) { /* first neighbour: N(phi(d)) */
+      s->set[d][u] = nv;
+      s->len[d][u] = or_below(nv, dv, ub);
+    } else {
+      s->len[d][u] = or_intersect(s->set[d - 1][u], s->len[d - 1][u], nv, dv, ub, s->buf[d][u]);
+      s->set[d][u] = s->buf[d][u];
+    }
+  }
+}
+
+static uint64_t or_dfs(or_state* s, int d) {
+  const or_plan* pl = s->pl;
+  int u = d + 1;
+  or_extend(s, d);
+  const uint32_t* c = s->set[d][u];
+  uint64_t n = s->len[d][u];
+  uint32_t ub = or_bound(s, u, d);
+  n = or_below(c, n, ub);
+  uint64_t total = 0;
+  if (u == pl->k - 1) {
+    if (pl->labeled) {
+      for (uint64_t i = 0; i < n; i++) total += or_admit(s, u, c[i]);
+    } else {
+      total = n;
+      uint32_t xm = pl->X[u];
+      for (int j = 0; j < u; j++)
+        if ((xm >> j & 1u) && or_member(c, n, s->phi[j])) total--;
+    }
+    return total;
+  }
+  for (uint64_t i = 0; i < n; i++) {
+    uint32_t x = c[i];
+    if (!or_admit(s, u, x)) continue;
+    s->phi[u] = x;
+    total += or_dfs(s, u);
+  }
+  return total;
+}
+
+/* job filter, src/job_queue.h:44-61 (labels ignored when unlabeled) */
+static int or_is_job(const or_graph* g, const or_plan* pl, uint32_t r, uint32_t c) {
+  if (!pl->labeled && pl->rc_order && !(r > c)) return 0;
+  if (pl->labeled) {
+    if ((1u << g->lab[r]) != pl->lbit[0] || (1u << g->lab[c]) != pl->lbit[1]) return 0;
+  }
+  if (or_deg(g, r) < (uint64_t)pl->deg[0] || or_deg(g, c) < (uint64_t)pl->deg[1]) return 0;
+  for (uint64_t e = g->rowptr[c]; e < g->rowptr[c + 1]; e++)
+    if (or_deg(g, g->colidx[e]) >= (uint64_t)pl->deg[2]) return 1;
+  return 0;
+}
+
+/*
+ * Count matches of pattern p in the CSR graph.
+ *   per_edge    optional u64[m]: count of the job rooted at CSR edge e (0 for
+ *               edges that are not jobs), before the |Aut| multiply.
+ *   edge_lo/hi  restrict the jobs to CSR edges [edge_lo, edge_hi) (hi=0: all).
+ *   threads     OpenMP threads (<=0: library default).
+ * Returns 0 on success and writes the total (after the |Aut| multiply) and
+ * the number of jobs.
+ */
+int or_count(uint64_t n, uint64_t m, const uint64_t* rowptr, const uint32_t* colidx,
+             const uint32_t* labels, const or_pattern* p, int labeled,
+             uint64_t edge_lo, uint64_t edge_hi, int threads,
+             uint64_t* per_edge, uint64_t* out_count, uint64_t* out_jobs) {
+  or_plan pl;
+  if (or_plan_pattern(p, labeled, &pl) != 0) return -1;
+  if (labeled && !labels) return -2;
+  if (labeled) {
+    for (uint64_t v = 0; v < n; v++) if (labels[v] > 31) return -3;
+  }
+  or_graph g = {n, rowptr, colidx, labeled ? labels : NULL};
+  uint64_t maxdeg = 0;
+  for (uint64_t v = 0; v < n; v++) if (or_deg(&g, (uint32_t)v) > maxdeg) maxdeg = or_deg(&g, (uint32_t)v);
+  if (edge_hi == 0 || edge_hi > m) edge_hi = m;
+  if (threads <= 0) threads = omp_get_max_threads();
+  uint64_t total = 0, njobs = 0;
+#pragma omp parallel num_threads(threads) reduction(+ : total, njobs)
+  {
+    or_state* s = (or_state*)calloc(1, sizeof(or_state));
+    s->g = &g;
+    s->pl = &pl;
+    for (int d = 0; d < OR_MAXK; d++)
+      for (int u = 0; u < OR_MAXK; u++) s->buf[d][u] = (uint32_t*)malloc(sizeof(uint32_t) * (maxdeg + 1));
+    /* edge chunks; the source vertex of the first edge by binary search,
+     * then walked forward */
+    const int64_t CH = 64;
+    int64_t nch = (int64_t)((edge_hi - edge_lo + CH - 1) / CH);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t ch = 0; ch < nch; ch++) {
+      uint64_t e0 = edge_lo + (uint64_t)ch * CH;
+      uint64_t e1 = e0 + CH < edge_hi ? e0 + CH : edge_hi;
+      uint64_t lo = 0, hi = n; /* largest r with rowptr[r] <= e0 */
+      while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (rowptr[mid] <= e0) lo = mid; else hi = mid;
+      }
+      uint64_t r = lo;
+      for (uint64_t e = e0; e < e1; e++) {
+        while (rowptr[r + 1] <= e) r++;
+        uint32_t c = colidx[e];
+        if (!or_is_job(&g, &pl, (uint32_t)r, c)) continue;
+        njobs++;
+        s->phi[0] = (uint32_t)r;
+        s->phi[1] = c;
+        /* depth 0: the running sets seeded by N(phi(0)) */
+        or_extend(s, 0);
+        uint64_t cnt = or_dfs(s, 1);
+        if (per_edge) per_edge[e] = cnt;
+        total += cnt;
+      }
+    }
+    for (int d = 0; d < OR_MAXK; d++)
+      for (int u = 0; u < OR_MAXK; u++) free(s->buf[d][u]);
+    free(s);
+  }
+  if (!labeled) total *= (uint64_t)pl.aut; /* test.cu:198 */
+  *out_count = total;
+  if (out_jobs) *out_jobs = njobs;
+  return 0;
+}
+
+/* Plan accessor for tests: fills aut, deg, A, B, X and rc_order. */
+int or_plan_info(const or_pattern* p, int labeled, long long* aut, int* deg,
+                 uint32_t* A, uint32_t* B, uint32_t* X, int* rc_order) {
+  or_plan pl;
+  if (or_plan_pattern(p, labeled, &pl) != 0) return -1;
+  *aut = pl.aut;
+  for (int i = 0; i < pl.k; i++) { deg[i] = pl.deg[i]; A[i] = pl.A[i]; B[i] = pl.B[i]; X[i] = pl.X[i]; }
+  *rc_order = pl.rc_order;
+  return 0;
+}

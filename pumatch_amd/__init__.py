@@ -1,0 +1,313 @@
+"""Python binding of libpumatch.so, the MI355X subgraph matcher.
+
+The library's C ABI (include/pumatch.h) is the product boundary; this module
+is a thin ctypes wrapper used by the tests and the benchmark.  It loads only
+the in-tree build (pumatch_amd/lib/libpumatch.so) and raises when it is
+missing: there is no CPU fallback.
+
+Host-side names follow the reference driver (test.cu): a Graph is what
+GraphPreprocessor loads (src/graph.h:228-254), a Pattern is what
+PatternPreprocessor plans (src/pattern.h:48-58), and match() is the timed
+launch of test.cu:171-200 returning (kernel ms, count).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpumatch.so")
+CLI_PATH = os.path.join(_HERE, "bin", "test")
+
+PM_GRAPH_LABELS = 0x1
+PM_GRAPH_HOST_ONLY = 0x2
+MAX_PATTERN = 7
+
+
+class PumatchError(RuntimeError):
+    pass
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("waves", C.c_int32),
+        ("chunk", C.c_int32),
+        ("repeats", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("edge_lo", C.c_uint64),
+        ("edge_hi", C.c_uint64),
+        ("per_edge", C.POINTER(C.c_uint64)),
+        ("stream", C.c_void_p),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("min_ms", C.c_double),
+        ("count", C.c_uint64),
+        ("jobs", C.c_uint64),
+        ("alg_bytes", C.c_uint64),
+        ("set_ops", C.c_uint64),
+        ("link_bytes", C.c_uint64),
+        ("waves", C.c_int32),
+        ("chunk", C.c_int32),
+    ]
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("m", C.c_uint64),
+        ("max_degree", C.c_uint32),
+        ("has_labels", C.c_int32),
+        ("on_device", C.c_int32),
+        ("reserved", C.c_int32),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class PatternInfo(C.Structure):
+    _fields_ = [
+        ("k", C.c_int32),
+        ("labeled", C.c_int32),
+        ("automorphisms", C.c_int64),
+        ("rc_order", C.c_int32),
+        ("nslots", C.c_int32),
+        ("degree", C.c_int32 * MAX_PATTERN),
+        ("label", C.c_int32 * MAX_PATTERN),
+        ("sym_mask", C.c_uint32 * MAX_PATTERN),
+        ("nbr_mask", C.c_uint32 * MAX_PATTERN),
+    ]
+
+
+# every symbol include/pumatch.h declares, with its ctypes signature
+_SIGS = {
+    "pm_last_error": (C.c_char_p, []),
+    "pm_version": (C.c_char_p, []),
+    "pm_graph_open": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pm_graph_from_csr": (C.c_int, [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pm_graph_generate_rmat": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_uint64, C.c_uint32,
+                                         C.POINTER(C.c_void_p)]),
+    "pm_graph_generate_chunglu": (C.c_int, [C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                            C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pm_graph_set_uniform_labels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64]),
+    "pm_graph_write": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pm_graph_info_get": (C.c_int, [C.c_void_p, C.POINTER(GraphInfo)]),
+    "pm_graph_host_csr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                    C.POINTER(C.c_void_p)]),
+    "pm_graph_to_device": (C.c_int, [C.c_void_p]),
+    "pm_graph_destroy": (None, [C.c_void_p]),
+    "pm_pattern_load": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pm_pattern_parse": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pm_pattern_info_get": (C.c_int, [C.c_void_p, C.POINTER(PatternInfo)]),
+    "pm_pattern_destroy": (None, [C.c_void_p]),
+    "pm_opts_default": (None, [C.POINTER(Opts)]),
+    "pm_match": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Result)]),
+    "pm_match_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts),
+                                 C.POINTER(Result)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libpumatch.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PumatchError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().pm_last_error()
+        raise PumatchError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+class Graph:
+    """A CSR graph owned by the library (host copy + HBM copy)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    # -- constructors --------------------------------------------------------
+    @classmethod
+    def open(cls, prefix: str, labels: bool = True, device: bool = True) -> "Graph":
+        h = C.c_void_p()
+        flags = (PM_GRAPH_LABELS if labels else 0) | (0 if device else PM_GRAPH_HOST_ONLY)
+        _check(lib().pm_graph_open(prefix.encode(), flags, C.byref(h)), f"open {prefix}")
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, rowptr, colidx, labels=None, device: bool = True) -> "Graph":
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.uint64)
+        colidx = np.ascontiguousarray(colidx, dtype=np.uint32)
+        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.uint32)
+        n = rowptr.size - 1
+        h = C.c_void_p()
+        flags = 0 if device else PM_GRAPH_HOST_ONLY
+        _check(lib().pm_graph_from_csr(n, colidx.size, rowptr.ctypes.data,
+                                       colidx.ctypes.data if colidx.size else None,
+                                       None if lab is None else lab.ctypes.data, flags,
+                                       C.byref(h)), "from_csr")
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: float = 16.0, a=0.57, b=0.19, c=0.19, seed: int = 1,
+             device: bool = True) -> "Graph":
+        h = C.c_void_p()
+        flags = 0 if device else PM_GRAPH_HOST_ONLY
+        _check(lib().pm_graph_generate_rmat(scale, edge_factor, a, b, c, seed, flags,
+                                            C.byref(h)), "rmat")
+        return cls(h.value)
+
+    @classmethod
+    def chung_lu(cls, n: int, gamma: float, dmax: float, avg: float, seed: int,
+                 device: bool = True) -> "Graph":
+        h = C.c_void_p()
+        flags = 0 if device else PM_GRAPH_HOST_ONLY
+        _check(lib().pm_graph_generate_chunglu(n, gamma, dmax, avg, seed, flags, C.byref(h)),
+               "chung_lu")
+        return cls(h.value)
+
+    # -- operations ----------------------------------------------------------
+    def set_uniform_labels(self, nlabels: int, seed: int) -> None:
+        _check(lib().pm_graph_set_uniform_labels(self._h, nlabels, seed), "labels")
+
+    def write(self, prefix: str) -> None:
+        _check(lib().pm_graph_write(self._h, prefix.encode()), f"write {prefix}")
+
+    def to_device(self) -> None:
+        _check(lib().pm_graph_to_device(self._h), "to_device")
+
+    def info(self) -> GraphInfo:
+        gi = GraphInfo()
+        _check(lib().pm_graph_info_get(self._h, C.byref(gi)), "info")
+        return gi
+
+    @property
+    def n(self) -> int:
+        return int(self.info().n)
+
+    @property
+    def m(self) -> int:
+        return int(self.info().m)
+
+    def csr(self):
+        """Zero-copy numpy views of the host CSR (rowptr u64, colidx u32,
+        labels u32 or None), valid while the graph lives."""
+        gi = self.info()
+        rp, ci, lb = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(lib().pm_graph_host_csr(self._h, C.byref(rp), C.byref(ci), C.byref(lb)), "csr")
+        rowptr = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_uint64)), shape=(gi.n + 1,))
+        colidx = (np.ctypeslib.as_array(C.cast(ci, C.POINTER(C.c_uint32)), shape=(gi.m,))
+                  if gi.m else np.zeros(0, np.uint32))
+        labels = (np.ctypeslib.as_array(C.cast(lb, C.POINTER(C.c_uint32)), shape=(gi.n,))
+                  if lb.value else None)
+        return rowptr, colidx, labels
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().pm_graph_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pattern:
+    """A planned query pattern (.g)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def load(cls, path: str, labeled: bool = True) -> "Pattern":
+        h = C.c_void_p()
+        _check(lib().pm_pattern_load(path.encode(), 1 if labeled else 0, C.byref(h)),
+               f"pattern {path}")
+        return cls(h.value)
+
+    @classmethod
+    def parse(cls, text: str, labeled: bool = True) -> "Pattern":
+        h = C.c_void_p()
+        _check(lib().pm_pattern_parse(text.encode(), 1 if labeled else 0, C.byref(h)),
+               "pattern")
+        return cls(h.value)
+
+    def info(self) -> PatternInfo:
+        pi = PatternInfo()
+        _check(lib().pm_pattern_info_get(self._h, C.byref(pi)), "pattern info")
+        return pi
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().pm_pattern_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class MatchResult:
+    kernel_ms: float
+    min_ms: float
+    count: int
+    jobs: int
+    alg_bytes: int
+    set_ops: int
+    link_bytes: int
+    waves: int
+    chunk: int
+    per_edge: np.ndarray | None = None
+
+
+def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int = 1,
+          edge_lo: int = 0, edge_hi: int = 0, per_edge: bool = False,
+          stream: int | None = None) -> MatchResult:
+    """Count matches of p in g on the GPU (pm_match)."""
+    o = Opts()
+    lib().pm_opts_default(C.byref(o))
+    o.waves, o.chunk, o.repeats = waves, chunk, repeats
+    o.edge_lo, o.edge_hi = edge_lo, edge_hi
+    pe = None
+    if per_edge:
+        pe = np.zeros(g.m, dtype=np.uint64)
+        o.per_edge = pe.ctypes.data_as(C.POINTER(C.c_uint64))
+    if stream is not None:
+        o.stream = stream
+    r = Result()
+    _check(lib().pm_match(g._h, p._h, C.byref(o), C.byref(r)), "match")
+    return MatchResult(r.kernel_ms, r.min_ms, int(r.count), int(r.jobs), int(r.alg_bytes),
+                       int(r.set_ops), int(r.link_bytes), int(r.waves), int(r.chunk), pe)
+
+
+def pattern_text(k: int, edges, labels=None) -> str:
+    """Write a pattern in the reference's .g layout: '# t 0', 'v i label'
+    lines, then every edge listed in both directions (pattern/*.g)."""
+    labels = labels if labels is not None else [1] * k
+    lines = ["# t 0"] + [f"v {i} {labels[i]}" for i in range(k)]
+    for a, b in edges:
+        lines.append(f"e {a} {b} 1")
+        lines.append(f"e {b} {a} 1")
+    return "\n".join(lines) + "\n"

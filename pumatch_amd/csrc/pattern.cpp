@@ -1,0 +1,219 @@
+// .g pattern parsing and the matching plan.
+//
+// The matching order is the file order (identity, src/pattern.h:104-122).
+// Automorphisms and the symmetry-breaking pairs follow the reference's
+// stabiliser chain (src/pattern.h:198-262).  The set-operation plan is this
+// library's own: for each depth d (vertices 0..d matched) and each future
+// vertex u > d, u's running candidate set is the intersection of N(phi(j))
+// over its earlier neighbours j <= d; identical masks share one slot, a slot
+// whose mask did not change is reused, and the first neighbour's list is a
+// zero-copy view of the CSR row.  The reference builds the same DAG of
+// intersections with copy slots (src/pattern.h:143-196, convert2oned
+// :329-355); the counts do not depend on how the intersections are shared.
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+
+// src/pattern.h:67-100: skip leading '#' lines, then "v id label" lines in
+// file order (the id field is ignored, labels are taken in line order), then
+// "e a b label" lines until end of file; an edge sets both adj[a][b] and
+// adj[b][a].  Blank or unparsable edge lines are rejected here (the
+// reference would read uninitialised values).
+int parse_pattern_text(const std::string& text, int labeled, Plan* out, std::string* err) {
+  std::istringstream in(text);
+  std::string line;
+  std::vector<std::string> lines;
+  while (std::getline(in, line)) {
+    while (!line.empty() && (line.back() == '\r' || line.back() == ' ' || line.back() == '\t'))
+      line.pop_back();
+    lines.push_back(line);
+  }
+  size_t i = 0;
+  while (i < lines.size() && !lines[i].empty() && lines[i][0] == '#') i++;
+  Plan p;
+  std::memset(p.adj, 0, sizeof(p.adj));
+  int k = 0;
+  while (i < lines.size() && !lines[i].empty() && lines[i][0] == 'v') {
+    std::istringstream s(lines[i]);
+    char tag;
+    long long id, lab;
+    if (!(s >> tag >> id >> lab)) {
+      *err = "malformed vertex line: '" + lines[i] + "'";
+      return PM_ERR_FORMAT;
+    }
+    if (k >= kMaxK) {
+      *err = "pattern has more than 7 vertices (PAT_SIZE)";
+      return PM_ERR_PATTERN;
+    }
+    if (labeled && (lab < 0 || lab > 31)) {
+      *err = "vertex label outside 0..31";
+      return PM_ERR_PATTERN;
+    }
+    p.vlabel[k] = labeled ? (int)lab : 1;
+    k++;
+    i++;
+  }
+  if (k == 0) {
+    *err = "no vertex lines";
+    return PM_ERR_FORMAT;
+  }
+  for (; i < lines.size(); i++) {
+    if (lines[i].empty()) continue;
+    if (lines[i][0] == '#') continue;
+    std::istringstream s(lines[i]);
+    char tag;
+    long long a, b, lab;
+    if (!(s >> tag >> a >> b >> lab) || tag != 'e') {
+      *err = "malformed edge line: '" + lines[i] + "'";
+      return PM_ERR_FORMAT;
+    }
+    if (a < 0 || b < 0 || a >= k || b >= k) {
+      *err = "edge endpoint outside the vertex range";
+      return PM_ERR_FORMAT;
+    }
+    p.adj[a][b] = (int)lab;
+    p.adj[b][a] = (int)lab;
+  }
+  p.k = k;
+  p.labeled = labeled ? 1 : 0;
+  *out = p;
+  return PM_OK;
+}
+
+namespace {
+
+// A permutation pp is kept when mapping every vertex i to pp[i] carries the
+// nonzero adjacency of i onto the label-1 adjacency of pp[i]
+// (src/pattern.h:207-234).
+bool keeps_adjacency(const Plan& p, const int* pp) {
+  uint32_t image[kMaxK] = {};
+  for (int i = 0; i < p.k; i++)
+    for (int j = 0; j < p.k; j++)
+      if (p.adj[i][j] != 0) image[pp[i]] |= 1u << pp[j];
+  for (int i = 0; i < p.k; i++) {
+    uint32_t ones = 0;
+    for (int j = 0; j < p.k; j++)
+      if (p.adj[i][j] == 1) ones |= 1u << j;
+    if (ones != image[i]) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+int build_plan(Plan* pp, std::string* err) {
+  Plan& p = *pp;
+  const int k = p.k;
+  if (k < 3) {
+    *err = "pattern needs at least 3 vertices";
+    return PM_ERR_PATTERN;
+  }
+  for (int i = 0; i < k; i++) {
+    int d = 0;
+    for (int j = 0; j < k; j++)
+      if (p.adj[i][j] > 0) d++;
+    p.deg[i] = d;  // src/pattern.h:115-121
+  }
+  for (int u = 0; u < k; u++) {
+    p.A[u] = p.X[u] = 0;
+    for (int j = 0; j < u; j++) {
+      if (p.adj[u][j] != 0) p.A[u] |= 1u << j;
+      else p.X[u] |= 1u << j;
+    }
+  }
+  if (!(p.A[1] & 1u)) {
+    *err = "pattern vertices 0 and 1 must be adjacent (a job is an edge)";
+    return PM_ERR_PATTERN;
+  }
+  for (int u = 2; u < k; u++)
+    if (p.A[u] == 0) {
+      *err = "every pattern vertex needs an earlier neighbour in file order";
+      return PM_ERR_PATTERN;
+    }
+
+  // automorphisms: all k! permutations in lexicographic order
+  std::vector<std::vector<int>> aut;
+  std::vector<int> perm(k);
+  for (int i = 0; i < k; i++) perm[i] = i;
+  do {
+    if (keeps_adjacency(p, perm.data())) aut.push_back(perm);
+  } while (std::next_permutation(perm.begin(), perm.end()));
+  p.aut = (long long)aut.size();
+
+  // stabiliser chain (src/pattern.h:238-252): at step v, every kept
+  // permutation that moves v contributes the pair (v, x[v]); those that fix
+  // v form the next group.  The pair (j,u) means phi(u) < phi(j).
+  for (int u = 0; u < k; u++) p.B[u] = 0;
+  std::vector<std::vector<int>> group = aut;
+  for (int v = 0; v < k; v++) {
+    std::vector<std::vector<int>> fixed;
+    for (auto& x : group) {
+      if (x[v] == v) fixed.push_back(x);
+      else p.B[x[v]] |= 1u << v;
+    }
+    group.swap(fixed);
+  }
+  if (p.labeled) {
+    for (int u = 0; u < k; u++) p.B[u] = 0;  // ub = INT_MAX, src/gpu_match.cu:454-458
+  }
+  p.rc_order = (!p.labeled && (p.B[1] & 1u)) ? 1 : 0;
+
+  // ---- slots ----
+  for (int d = 0; d < kMaxK; d++) {
+    p.ops[d].clear();
+    for (int u = 0; u < kMaxK; u++) p.slot_of[d][u] = -1;
+  }
+  p.nslots = 0;
+  p.n_materialised = 0;
+  for (int d = 0; d <= k - 2; d++) {
+    const uint32_t upto = (2u << d) - 1u;
+    for (int u = d + 1; u < k; u++) {
+      const uint32_t M = p.A[u] & upto;
+      if (M == 0) continue;
+      if (!(M >> d & 1u)) {
+        p.slot_of[d][u] = p.slot_of[d - 1][u];
+        continue;
+      }
+      // share with an earlier u of the same mask at this depth
+      int found = -1;
+      for (int w = d + 1; w < u; w++)
+        if ((p.A[w] & upto) == M && p.slot_of[d][w] >= 0) found = p.slot_of[d][w];
+      if (found >= 0) {
+        p.slot_of[d][u] = found;
+        continue;
+      }
+      if (p.nslots >= kMaxSlots) {
+        *err = "too many slots";
+        return PM_ERR_PATTERN;
+      }
+      SlotOp op;
+      std::memset(&op, 0, sizeof(op));
+      op.slot = (int8_t)p.nslots++;
+      if (M == (1u << d)) {
+        op.kind = 0;
+        op.parent = -1;
+      } else {
+        op.kind = 1;
+        op.parent = (int8_t)p.slot_of[d - 1][u];
+        if (d < k - 2) p.n_materialised++;
+      }
+      op.labmask = 0;
+      op.ncons = 0;
+      for (int w = d + 1; w < k; w++) {
+        if ((p.A[w] & upto) != M) continue;
+        op.labmask |= 1u << p.vlabel[w];
+        op.cons_b[op.ncons++] = (uint8_t)(p.B[w] & upto);
+      }
+      p.slot_of[d][u] = op.slot;
+      p.ops[d].push_back(op);
+    }
+  }
+  p.last_inline = !p.ops[k - 2].empty();
+  return PM_OK;
+}
+
+}  // namespace pm

@@ -1,0 +1,110 @@
+// Internal types of the MI355X subgraph matcher (host side).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pumatch.h"
+
+namespace pm {
+
+constexpr int kMaxK = PM_MAX_PATTERN;  // PAT_SIZE, src/config.h:9
+constexpr int kMaxSlots = 32;
+
+// Error reporting: the last error message of the calling thread.
+void set_error(const std::string& msg);
+
+// Host CSR in the reference's on-disk layout (src/graph.h:228-254):
+// rowptr u64[n+1], colidx u32[m], raw labels u32[n].
+struct HostGraph {
+  uint64_t n = 0, m = 0;
+  std::vector<uint64_t> rowptr;
+  std::vector<uint32_t> colidx;
+  std::vector<uint32_t> labels;  // empty when the graph carries no labels
+  uint32_t max_degree = 0;
+};
+
+// Device-resident copy (HBM).  label8 holds the raw label (0..31) as a byte,
+// maxnbr[v] = max degree over N(v) (evaluates the job filter's
+// "exists w in N(c) with deg(w) >= d_P(2)" in O(1)).
+struct DeviceGraph {
+  uint64_t* rowptr = nullptr;
+  uint32_t* colidx = nullptr;
+  uint8_t* label8 = nullptr;
+  uint32_t* maxnbr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace pm
+
+struct pm_graph {
+  pm::HostGraph h;
+  pm::DeviceGraph d;
+  bool on_device = false;
+  int device = 0;
+};
+
+namespace pm {
+
+// One set operation executed when vertex `depth` has just been matched.
+//   kind VIEW:      slot = N(phi(depth))               (zero-copy CSR view)
+//   kind INTERSECT: slot = slot[parent] n N(phi(depth)) (materialised)
+struct SlotOp {
+  int8_t kind;      // 0 = VIEW, 1 = INTERSECT
+  int8_t slot;      // destination slot id
+  int8_t parent;    // parent slot id (INTERSECT)
+  int8_t pad;
+  uint32_t labmask;  // labeled: OR of 1<<label over consumers (filter on materialise)
+  // symmetry-breaking truncation: slot keeps x < max over consumers c of
+  // min_{j in cons_b[c]} phi(j); a consumer with an empty mask disables it.
+  uint8_t ncons;
+  uint8_t cons_b[kMaxK];
+};
+
+struct Plan {
+  int k = 0;
+  int labeled = 1;
+  long long aut = 1;             // PatternMultiplicity, src/pattern.h:236
+  int rc_order = 0;              // unlabeled job filter r > c
+  int deg[kMaxK] = {};           // pattern degree
+  int vlabel[kMaxK] = {};        // pattern vertex label
+  uint32_t A[kMaxK] = {};        // earlier neighbours
+  uint32_t B[kMaxK] = {};        // symmetry pairs phi(u) < phi(j)
+  uint32_t X[kMaxK] = {};        // earlier non-neighbours
+  int adj[kMaxK][kMaxK] = {};    // edge labels as read
+  // execution plan
+  int nslots = 0;
+  int slot_of[kMaxK][kMaxK];     // [depth][u] slot holding u's running set
+  std::vector<SlotOp> ops[kMaxK];  // ops run at depth d (d = 0..k-2)
+  int n_materialised = 0;        // INTERSECT slots (arena sizing)
+  int last_inline = 0;           // last vertex counted by an inline intersect
+};
+
+}  // namespace pm
+
+struct pm_pattern {
+  pm::Plan plan;
+  std::string path;
+};
+
+namespace pm {
+
+int parse_pattern_text(const std::string& text, int labeled, Plan* out, std::string* err);
+int build_plan(Plan* p, std::string* err);
+
+// generators (gen.cpp)
+int generate_rmat(HostGraph* g, int scale, double edge_factor, double a, double b, double c,
+                  uint64_t seed);
+int generate_chunglu(HostGraph* g, uint64_t n, double gamma, double dmax, double avg,
+                     uint64_t seed);
+void assign_uniform_labels(HostGraph* g, uint32_t nlabels, uint64_t seed);
+int build_csr_from_pairs(HostGraph* g, uint64_t n, std::vector<uint64_t>& pairs);
+void finalize_graph(HostGraph* g);
+void compute_max_neighbour_degree(const HostGraph& g, std::vector<uint32_t>* out);
+
+// device (match.hip)
+int device_upload(pm_graph* g);
+void device_release(pm_graph* g);
+int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res);
+
+}  // namespace pm
