@@ -44,10 +44,14 @@ typedef struct pm_graph pm_graph;
 typedef struct pm_pattern pm_pattern;
 
 typedef struct {
-  int32_t waves;        /* persistent wavefronts (0 = auto: 8 per CU) */
+  int32_t waves;        /* persistent wavefronts (0 = auto: 16 per CU) */
   int32_t chunk;        /* jobs (CSR edges) claimed per atomic (0 = auto) */
   int32_t repeats;      /* timed launches; kernel_ms is their mean (0 = 1) */
-  int32_t reserved0;
+  int32_t split;        /* heavy-root split: vertex-2 candidates per range
+                           task (0 = auto 64, < 0 = no splitting) */
+  int32_t hub_factor;   /* fused tail: candidates whose list is longer than
+                           hub_factor x the shared set go one by one (0 = 8) */
+  int32_t reserved;
   uint64_t edge_lo;     /* restrict jobs to CSR edges [edge_lo, edge_hi); hi=0: all */
   uint64_t edge_hi;
   uint64_t* per_edge;   /* optional host u64[m]: count per job edge (before x|Aut|) */
@@ -62,8 +66,13 @@ typedef struct {
   uint64_t alg_bytes;   /* algorithmic bytes of one launch (see DESIGN) */
   uint64_t set_ops;     /* set operations executed by one launch */
   uint64_t link_bytes;  /* host->HBM bytes paged during the launch */
+  uint64_t tasks;       /* range tasks created by heavy-root splitting */
   int32_t waves;        /* wavefronts actually launched */
   int32_t chunk;
+  /* per kernel phase of the last timed launch (hipEvents on the launch
+     stream): 0 pendant-tree folding, 1 edge phase, 2 range-task phase */
+  double phase_ms[3];
+  uint64_t phase_alg_bytes[3];
 } pm_result;
 
 typedef struct {

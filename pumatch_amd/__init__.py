@@ -36,7 +36,9 @@ class Opts(C.Structure):
         ("waves", C.c_int32),
         ("chunk", C.c_int32),
         ("repeats", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("split", C.c_int32),
+        ("hub_factor", C.c_int32),
+        ("reserved", C.c_int32),
         ("edge_lo", C.c_uint64),
         ("edge_hi", C.c_uint64),
         ("per_edge", C.POINTER(C.c_uint64)),
@@ -53,8 +55,11 @@ class Result(C.Structure):
         ("alg_bytes", C.c_uint64),
         ("set_ops", C.c_uint64),
         ("link_bytes", C.c_uint64),
+        ("tasks", C.c_uint64),
         ("waves", C.c_int32),
         ("chunk", C.c_int32),
+        ("phase_ms", C.c_double * 3),
+        ("phase_alg_bytes", C.c_uint64 * 3),
     ]
 
 
@@ -277,18 +282,22 @@ class MatchResult:
     alg_bytes: int
     set_ops: int
     link_bytes: int
+    tasks: int
     waves: int
     chunk: int
+    phase_ms: tuple = ()
+    phase_alg_bytes: tuple = ()
     per_edge: np.ndarray | None = None
 
 
 def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int = 1,
-          edge_lo: int = 0, edge_hi: int = 0, per_edge: bool = False,
-          stream: int | None = None) -> MatchResult:
+          split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
+          per_edge: bool = False, stream: int | None = None) -> MatchResult:
     """Count matches of p in g on the GPU (pm_match)."""
     o = Opts()
     lib().pm_opts_default(C.byref(o))
     o.waves, o.chunk, o.repeats = waves, chunk, repeats
+    o.split, o.hub_factor = split, hub_factor
     o.edge_lo, o.edge_hi = edge_lo, edge_hi
     pe = None
     if per_edge:
@@ -299,7 +308,8 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
     r = Result()
     _check(lib().pm_match(g._h, p._h, C.byref(o), C.byref(r)), "match")
     return MatchResult(r.kernel_ms, r.min_ms, int(r.count), int(r.jobs), int(r.alg_bytes),
-                       int(r.set_ops), int(r.link_bytes), int(r.waves), int(r.chunk), pe)
+                       int(r.set_ops), int(r.link_bytes), int(r.tasks), int(r.waves), int(r.chunk),
+                       tuple(r.phase_ms), tuple(int(x) for x in r.phase_alg_bytes), pe)
 
 
 def pattern_text(k: int, edges, labels=None) -> str:

@@ -1,20 +1,38 @@
 // Subgraph-matching kernels for gfx950 (CDNA4, wave64).
 //
-// One persistent wavefront runs one depth-first search at a time
-// (the reference: one warp per DFS, _parallel_match / match / extend,
-// src/gpu_match.cu:855-1178).  Jobs are CSR edges (r,c) claimed in chunks
-// with one global atomic (replaces the spin-locked job array of
-// src/gpu_match.cu:321-384 and the host job scan src/job_queue.h:39-89; the
-// job filter of src/job_queue.h:44-61 is evaluated lane-parallel on the
-// device).  Set operations are wave-cooperative sorted-list intersections:
-// each lane takes one element of the shorter list, searches the longer one
-// inside a window that advances monotonically, and the hits are compacted in
-// order with a 64-bit ballot + popcount (src/gpu_match.cu:238-319 does the
-// same with 32-bit ballots and a binary search over the whole list).
-// Candidate batches of 64 are staged in LDS together with the DFS stack.
-// Intermediate sets live in a per-wave HBM arena sized so no set is ever
-// truncated (the reference caps every set at 4096 entries,
+// One persistent wavefront runs one depth-first search at a time (the
+// reference: one warp per DFS, _parallel_match / match / extend,
+// src/gpu_match.cu:855-1178).
+//
+// Work distribution (replaces the spin-locked job array of
+// src/gpu_match.cu:321-384 and the host job scan src/job_queue.h:39-89):
+//   phase 1  persistent waves claim chunks of CSR edges with one global
+//            atomic; the job filter of src/job_queue.h:44-61 is evaluated
+//            lane-parallel; a job whose vertex-2 candidate list is longer
+//            than `split` keeps the first block and appends the rest as
+//            range tasks (edge, [lo,hi)) to a task buffer;
+//   phase 2  persistent waves claim range tasks, rebuild the depth-0/1 sets
+//            of the task's edge and finish the range.  This is the heavy-
+//            root splitting the reference approximates with stealing
+//            (trans_layer, src/gpu_match.cu:21-109, disabled by default).
+//
+// Set operations (src/gpu_match.cu:238-319): sorted-list intersections,
+// each lane taking one element of the shorter list and searching the longer
+// one inside a monotonically advancing window; hits are compacted in order
+// with a 64-bit ballot + popcount.  Intermediate sets live in a per-wave HBM
+// arena sized so no set is ever truncated (the reference caps sets at 4096,
 // src/gpu_match.cu:302,316).
+//
+// The last two core vertices are fused (the reference enumerates vertex k-2
+// one candidate at a time and counts vertex k-1, :1018-1041): all candidates
+// of vertex k-2 are taken 64 at a time and the matches of vertex k-1 for
+// each are counted lane-parallel; when that count is an intersection with a
+// shared parent set P, the candidates' neighbour lists are walked
+// edge-parallel and membership in P is an LDS hash probe.
+//
+// Labeled (homomorphism) counting folds pendant trees of the pattern into
+// per-vertex weight tables built by an edge-parallel sparse pass (k_fold);
+// the DFS runs over the remaining core and multiplies weights.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,9 +49,12 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr uint32_t kInf = 0xFFFFFFFFu;
+constexpr int kHashBits = 10;
+constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 struct KOp {
-  int8_t kind;   // 0 VIEW, 1 INTERSECT
+  int8_t kind;  // 0 VIEW, 1 INTERSECT
   int8_t slot;
   int8_t parent;
   int8_t ncons;
@@ -42,19 +63,23 @@ struct KOp {
 };
 
 struct KPlan {
-  int32_t k, labeled, rc_order, nops;
-  uint32_t deg0, deg1, deg2, lab0, lab1;
-  uint32_t lab[8];        // label of pattern vertex u
-  uint32_t B[8], X[8];
+  int32_t k;  // core size
+  int32_t labeled, rc_order, nops;
+  uint32_t deg0, deg1, deg2, lab0, lab1;  // job filter (whole pattern)
+  uint32_t lab[8];                          // core vertex labels
+  uint32_t B[8], X[8];                      // core symmetry / non-neighbour masks
   int8_t op_begin[8], op_end[8];
-  int8_t cand_slot[8];    // [d]: slot holding vertex d+1's candidates
-  int8_t cand_check[8];   // [d]: labeled, candidates still need the label test
-  int8_t region[32];      // slot -> arena region (-1: not materialised)
-  int8_t last_kind;       // 0 reuse slot last_src, 1 view of N(phi(k-2)), 2 inline intersect
+  int8_t cand_slot[8];   // [d]: slot holding candidates of core vertex d+1
+  int8_t cand_check[8];  // [d]: labeled, candidates still need the label test
+  int8_t region[kMaxSlots];
+  int8_t last_kind;   // 0 reuse slot last_src, 1 view N(phi(k-2)), 2 intersect
   int8_t last_src;
-  int8_t last_check;
+  int8_t last_check;  // labeled: label test needed on the last vertex
   int8_t pad;
+  uint32_t split;       // phase-1 split threshold on vertex-2 candidates
+  uint32_t hub_factor;  // fused: owners with deg > hub_factor*|P| go one by one
   KOp ops[kMaxSlots];
+  const unsigned long long* wt[8];  // weight table per core vertex (nullptr = 1)
 };
 
 struct KGraph {
@@ -65,17 +90,32 @@ struct KGraph {
   const uint32_t* maxnbr;
 };
 
+struct Task {
+  unsigned long long edge;
+  uint32_t lo, hi;
+};
+
 struct WaveState {
   const uint32_t* sptr[kMaxSlots];
   uint32_t slen[kMaxSlots];
   uint32_t phi[8];
-  uint32_t cbase[8];
+  unsigned long long wacc[8];
   uint32_t cnext[8];
   uint32_t clen[8];
   uint64_t cmask[8];
   uint32_t cval[8][kWave];
   uint32_t jr[kWave];
   uint32_t jc[kWave];
+  // fused step: per-owner data of the current batch
+  uint32_t own_off[kWave + 1];
+  uint32_t own_ub[kWave];
+  unsigned long long own_w[kWave];
+  uint64_t own_rp[kWave];
+  uint32_t hash[kHashCap];
+};
+
+struct Counters {
+  uint64_t alg = 0, nops = 0;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -87,6 +127,11 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x) {
 template <typename T>
 __device__ __forceinline__ T* uniptr(T* p) {
   return (T*)uni64((uint64_t)p);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
 }
 
 // number of elements of a[0..n) strictly below x (src/gpu_match.cu:185-202)
@@ -104,17 +149,33 @@ __device__ __forceinline__ uint32_t below(const uint32_t* a, uint32_t n, uint32_
   return lo;
 }
 
+__device__ __forceinline__ bool member(const uint32_t* a, uint32_t n, uint32_t x) {
+  uint32_t p = below(a, n, x);
+  return p < n && a[p] == x;
+}
+
 __device__ __forceinline__ uint32_t degree(const KGraph& g, uint32_t v) {
   return (uint32_t)(g.rowptr[v + 1] - g.rowptr[v]);
 }
 
+__device__ __forceinline__ unsigned long long wt_of(const KPlan& P, int u, uint32_t x) {
+  const unsigned long long* t = P.wt[u];
+  return t ? t[x] : 1ull;
+}
+
+__device__ __forceinline__ uint32_t hslot(uint32_t y) {
+  return (y * 0x9E3779B1u) >> (32 - kHashBits);
+}
+
 // Wave-cooperative intersection of two sorted lists (already truncated).
-// WRITE: store the hits in order at out.  LABEL: keep x only when bit
-// lab[x] of labmask is set.  Returns the number of kept elements.
-template <bool WRITE, bool LABEL>
+// WRITE: store the hits in order at out.  LABEL: keep y only when bit lab[y]
+// of labmask is set.  WEIGHT: also sum wt[y] (1 when wt is null) over the
+// hits into *wsum.
+template <bool WRITE, bool LABEL, bool WEIGHT>
 __device__ uint32_t wave_isect(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
-                               uint32_t labmask, const uint8_t* __restrict__ lab, uint32_t* out,
-                               uint32_t lane, uint64_t& alg) {
+                               uint32_t labmask, const uint8_t* __restrict__ lab,
+                               const unsigned long long* __restrict__ wt, uint32_t* out,
+                               uint32_t lane, Counters& ct, unsigned long long* wsum) {
   if (na > nb) {
     const uint32_t* t = a;
     a = b;
@@ -123,8 +184,9 @@ __device__ uint32_t wave_isect(const uint32_t* a, uint32_t na, const uint32_t* b
     na = nb;
     nb = tn;
   }
-  alg += 4ull * (na + nb);
+  ct.alg += 4ull * (na + nb);
   uint32_t cnt = 0, lo = 0;
+  unsigned long long ws = 0;
   const uint64_t lt = (1ull << lane) - 1ull;
   for (uint32_t base = 0; base < na && lo < nb; base += kWave) {
     const uint32_t i = base + lane;
@@ -135,6 +197,7 @@ __device__ uint32_t wave_isect(const uint32_t* a, uint32_t na, const uint32_t* b
       pos = lo + below(b + lo, nb - lo, x);
       hit = pos < nb && b[pos] == x;
       if (LABEL && hit) hit = (labmask >> lab[x]) & 1u;
+      if (WEIGHT && hit) ws += wt ? wt[x] : 1ull;
     }
     const uint64_t m = __ballot(hit);
     if (WRITE && hit) out[cnt + __popcll(m & lt)] = x;
@@ -142,25 +205,31 @@ __device__ uint32_t wave_isect(const uint32_t* a, uint32_t na, const uint32_t* b
     const uint32_t last = min(na - base, (uint32_t)kWave) - 1u;
     lo = uni((uint32_t)__shfl((int)pos, (int)last));
   }
-  if (WRITE) alg += 4ull * cnt;
+  if (WRITE) ct.alg += 4ull * cnt;
+  if (WEIGHT) *wsum = wave_sum(ws);
   return cnt;
 }
 
-// Count x in a[0..n) with lab[x] == want (wave-parallel).
-__device__ uint32_t wave_count_label(const uint32_t* a, uint32_t n, uint32_t want,
-                                     const uint8_t* __restrict__ lab, uint32_t lane, uint64_t& alg) {
-  uint32_t cnt = 0;
+// Sum over y in a[0..n) with lab[y] == want (all y when !check) of wt[y]
+// (1 when wt is null): wave-parallel, uniform result.
+__device__ unsigned long long wave_label_sum(const uint32_t* a, uint32_t n, uint32_t want,
+                                             bool check, const uint8_t* __restrict__ lab,
+                                             const unsigned long long* __restrict__ wt,
+                                             uint32_t lane, Counters& ct) {
+  unsigned long long s = 0;
   for (uint32_t base = 0; base < n; base += kWave) {
     const uint32_t i = base + lane;
-    bool hit = i < n && lab[a[i]] == want;
-    cnt += (uint32_t)__popcll(__ballot(hit));
+    if (i < n) {
+      const uint32_t y = a[i];
+      if (!check || lab[y] == want) s += wt ? wt[y] : 1ull;
+    }
   }
-  alg += 5ull * n;
-  return cnt;
+  ct.alg += (4ull + (check ? 1 : 0) + (wt ? 8 : 0)) * n;
+  return wave_sum(s);
 }
 
-// truncation bound of a slot: max over consumers of min over their
-// symmetry partners matched so far; kInf when any consumer is unconstrained.
+// slot truncation: max over consumers of min over their symmetry partners
+// matched so far; kInf when any consumer is unconstrained
 __device__ __forceinline__ uint32_t slot_bound(const KOp& op, const WaveState& ws) {
   uint32_t ub = 0;
   for (int c = 0; c < op.ncons; c++) {
@@ -177,8 +246,10 @@ __device__ __forceinline__ uint32_t slot_bound(const KOp& op, const WaveState& w
   return ub;
 }
 
-__device__ __forceinline__ uint32_t vertex_bound(const KPlan& P, const WaveState& ws, int u) {
-  uint32_t b = P.B[u], ub = kInf;
+// bound of core vertex u from its symmetry partners among 0..upto
+__device__ __forceinline__ uint32_t vertex_bound(const KPlan& P, const WaveState& ws, int u,
+                                                 int upto) {
+  uint32_t b = P.B[u] & ((2u << upto) - 1u), ub = kInf;
   while (b) {
     int j = __builtin_ctz(b);
     b &= b - 1;
@@ -187,9 +258,8 @@ __device__ __forceinline__ uint32_t vertex_bound(const KPlan& P, const WaveState
   return ub;
 }
 
-// run the slot operations of depth d (vertex d just matched)
 __device__ void run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, uint32_t* arena,
-                        uint32_t cap, uint32_t lane, uint64_t& alg, uint64_t& nops) {
+                        uint32_t cap, uint32_t lane, Counters& ct) {
   const uint32_t v = uni(ws.phi[d]);
   const uint32_t* nv = uniptr(g.colidx + g.rowptr[v]);
   const uint32_t dv = uni(degree(g, v));
@@ -207,32 +277,48 @@ __device__ void run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, u
       uint32_t* out = arena + (size_t)P.region[op.slot] * cap;
       uint32_t cnt;
       if (P.labeled)
-        cnt = wave_isect<true, true>(a, na, nv, nb, op.labmask, g.lab, out, lane, alg);
+        cnt = wave_isect<true, true, false>(a, na, nv, nb, op.labmask, g.lab, nullptr, out, lane,
+                                            ct, nullptr);
       else
-        cnt = wave_isect<true, false>(a, na, nv, nb, 0, g.lab, out, lane, alg);
+        cnt = wave_isect<true, false, false>(a, na, nv, nb, 0, g.lab, nullptr, out, lane, ct,
+                                             nullptr);
       ws.sptr[op.slot] = out;
       ws.slen[op.slot] = uni(cnt);
-      nops++;
+      ct.nops++;
     }
   }
 }
 
+// candidates of core vertex d+1 (truncated by its full symmetry bound)
 __device__ __forceinline__ void setup_cand(const KPlan& P, WaveState& ws, int d) {
   const int s = P.cand_slot[d];
   uint32_t n = uni(ws.slen[s]);
   if (!P.labeled) {
-    const uint32_t ub = vertex_bound(P, ws, d + 1);
+    const uint32_t ub = vertex_bound(P, ws, d + 1, d);
     if (ub != kInf) n = uni(below(uniptr(ws.sptr[s]), n, ub));
   }
   ws.clen[d] = n;
   ws.cnext[d] = 0;
-  ws.cbase[d] = 0;
   ws.cmask[d] = 0;
 }
 
-// next admitted candidate of vertex d+1; false when exhausted
+// label / injectivity test of candidate x for core vertex u (lane-local)
+__device__ __forceinline__ bool admit(const KGraph& g, const KPlan& P, const WaveState& ws, int u,
+                                      bool check, uint32_t x) {
+  if (P.labeled) return !check || g.lab[x] == P.lab[u];
+  uint32_t xm = P.X[u];
+  bool ok = true;
+  while (xm) {
+    int j = __builtin_ctz(xm);
+    xm &= xm - 1;
+    ok = ok && (x != ws.phi[j]);
+  }
+  return ok;
+}
+
+// next admitted candidate of core vertex d+1; false when exhausted
 __device__ __forceinline__ bool next_cand(const KGraph& g, const KPlan& P, WaveState& ws, int d,
-                                          uint32_t lane, uint32_t& x, uint64_t& alg) {
+                                          uint32_t lane, uint32_t& x, Counters& ct) {
   for (;;) {
     uint64_t m = uni64(ws.cmask[d]);
     if (m) {
@@ -244,39 +330,29 @@ __device__ __forceinline__ bool next_cand(const KGraph& g, const KPlan& P, WaveS
     const uint32_t base = uni(ws.cnext[d]);
     const uint32_t n = uni(ws.clen[d]);
     if (base >= n) return false;
-    ws.cbase[d] = base;
     ws.cnext[d] = base + kWave;
     const uint32_t* c = uniptr(ws.sptr[P.cand_slot[d]]);
     const uint32_t idx = base + lane;
     bool ok = idx < n;
     uint32_t v = 0;
-    const int u = d + 1;
     if (ok) {
       v = c[idx];
-      if (P.labeled) {
-        if (P.cand_check[d]) ok = g.lab[v] == P.lab[u];
-      } else {
-        uint32_t xm = P.X[u];
-        while (xm) {
-          int j = __builtin_ctz(xm);
-          xm &= xm - 1;
-          ok = ok && (v != ws.phi[j]);
-        }
-      }
+      ok = admit(g, P, ws, d + 1, P.cand_check[d], v);
     }
     ws.cval[d][lane] = v;
-    alg += 4ull * min(n - base, (uint32_t)kWave);
+    ct.alg += 4ull * min(n - base, (uint32_t)kWave);
     ws.cmask[d] = __ballot(ok);
     __syncthreads();
   }
 }
 
-// matches of the last pattern vertex k-1 once vertices 0..k-2 are fixed
-__device__ uint64_t last_count(const KGraph& g, const KPlan& P, WaveState& ws, uint32_t lane,
-                               uint64_t& alg, uint64_t& nops) {
+// Weighted count of the last core vertex once 0..k-2 are fixed (one node).
+__device__ unsigned long long last_count(const KGraph& g, const KPlan& P, WaveState& ws,
+                                         uint32_t lane, Counters& ct) {
   const int u = P.k - 1;
   const int dl = P.k - 2;
-  const uint32_t ub = P.labeled ? kInf : vertex_bound(P, ws, u);
+  const uint32_t ub = P.labeled ? kInf : vertex_bound(P, ws, u, dl);
+  const unsigned long long* w2 = P.wt[u];
   const uint32_t* a;
   uint32_t na;
   if (P.last_kind == 0) {
@@ -290,40 +366,310 @@ __device__ uint64_t last_count(const KGraph& g, const KPlan& P, WaveState& ws, u
   if (ub != kInf) na = uni(below(a, na, ub));
   if (P.last_kind != 2) {
     if (P.labeled) {
-      if (!P.last_check) return na;
-      nops++;
-      return uni(wave_count_label(a, na, P.lab[u], g.lab, lane, alg));
+      if (!P.last_check && !w2) return na;
+      ct.nops++;
+      return wave_label_sum(a, na, P.lab[u], P.last_check, g.lab, w2, lane, ct);
     }
-    // injectivity: drop earlier non-neighbours that sit in the list
-    uint32_t drop = 0;
     const uint32_t xm = P.X[u];
     bool in = false;
-    if (lane < (uint32_t)u && ((xm >> lane) & 1u)) {
-      const uint32_t y = ws.phi[lane];
-      const uint32_t p = below(a, na, y);
-      in = p < na && a[p] == y;
-    }
-    drop = (uint32_t)__popcll(__ballot(in));
-    return (uint64_t)(na - drop);
+    if (lane < (uint32_t)u && ((xm >> lane) & 1u)) in = member(a, na, ws.phi[lane]);
+    return (unsigned long long)(na - (uint32_t)__popcll(__ballot(in)));
   }
-  // inline intersection of the parent slot with N(phi(k-2)), count only
   const uint32_t* p = uniptr(ws.sptr[P.last_src]);
   uint32_t np = uni(ws.slen[P.last_src]);
   if (ub != kInf) np = uni(below(p, np, ub));
-  nops++;
+  ct.nops++;
   if (P.labeled) {
-    return uni(wave_isect<false, true>(p, np, a, na, 1u << P.lab[u], g.lab, nullptr, lane, alg));
+    unsigned long long s = 0;
+    uint32_t c = wave_isect<false, true, true>(p, np, a, na,
+                                               P.last_check ? (1u << P.lab[u]) : 0xFFFFFFFFu,
+                                               g.lab, w2, nullptr, lane, ct, &s);
+    return w2 ? s : (unsigned long long)c;
   }
-  uint32_t cnt = uni(wave_isect<false, false>(p, np, a, na, 0, g.lab, nullptr, lane, alg));
+  uint32_t cnt = uni(wave_isect<false, false, false>(p, np, a, na, 0, g.lab, nullptr, nullptr,
+                                                     lane, ct, nullptr));
   const uint32_t xm = P.X[u];
   bool in = false;
   if (lane < (uint32_t)u && ((xm >> lane) & 1u)) {
     const uint32_t y = ws.phi[lane];
-    const uint32_t q1 = below(a, na, y);
-    const uint32_t q2 = below(p, np, y);
-    in = q1 < na && a[q1] == y && q2 < np && p[q2] == y;
+    in = member(a, na, y) && member(p, np, y);
   }
   cnt -= (uint32_t)__popcll(__ballot(in));
+  return cnt;
+}
+
+// The last two core vertices u1 = d+1, u2 = d+2 (d = k-3), fused: weighted
+// number of matches of (u1, u2) over the candidates of u1 in
+// [cnext[d], clen[d]).  Uniform result.
+__device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveState& ws, int d,
+                                         uint32_t lane, Counters& ct) {
+  const int u1 = d + 1, u2 = d + 2;
+  const bool lab = P.labeled;
+  const unsigned long long wd = uni64(ws.wacc[d]);
+  const unsigned long long* w1 = P.wt[u1];
+  const unsigned long long* w2 = P.wt[u2];
+  const uint32_t* cand = uniptr(ws.sptr[P.cand_slot[d]]);
+  const uint32_t cbeg = uni(ws.cnext[d]), cend = uni(ws.clen[d]);
+  // u2's bound from vertices 0..d, and whether phi(u2) < phi(u1)
+  const uint32_t ub_fixed = lab ? kInf : vertex_bound(P, ws, u2, d);
+  const bool sym_x = !lab && ((P.B[u2] >> u1) & 1u);
+  const uint32_t dropm = lab ? 0u : (P.X[u2] & ((2u << d) - 1u));
+  const bool x_drop = !lab && ((P.X[u2] >> u1) & 1u);
+  unsigned long long acc = 0, dropc = 0;
+
+  if (P.last_kind == 0) {
+    // u2's set does not depend on u1's image
+    const uint32_t* S = uniptr(ws.sptr[P.last_src]);
+    const uint32_t nS = uni(ws.slen[P.last_src]);
+    unsigned long long Lc = 0;
+    uint32_t inS = 0;  // bit j: phi(j) in S (j in dropm)
+    if (lab) {
+      Lc = (!P.last_check && !w2)
+               ? (unsigned long long)nS
+               : wave_label_sum(S, nS, P.lab[u2], P.last_check, g.lab, w2, lane, ct);
+    } else {
+      bool in = false;
+      if (lane <= (uint32_t)d && ((dropm >> lane) & 1u)) in = member(S, nS, ws.phi[lane]);
+      inS = (uint32_t)__ballot(in);
+    }
+    for (uint32_t base = cbeg; base < cend; base += kWave) {
+      const uint32_t i = base + lane;
+      if (i < cend) {
+        const uint32_t x = cand[i];
+        if (admit(g, P, ws, u1, P.cand_check[d], x)) {
+          if (lab) {
+            acc += wd * (w1 ? w1[x] : 1ull) * Lc;
+          } else {
+            const uint32_t ub = sym_x ? min(ub_fixed, x) : ub_fixed;
+            const uint32_t c = below(S, nS, ub);
+            uint32_t dr = 0, b = inS;
+            while (b) {
+              int j = __builtin_ctz(b);
+              b &= b - 1;
+              dr += ws.phi[j] < ub;
+            }
+            if (x_drop && member(S, c, x)) dr++;
+            acc += c - dr;
+          }
+        }
+      }
+    }
+    ct.alg += 8ull * (cend - cbeg);
+    ct.nops++;
+    return wave_sum(acc);
+  }
+
+  if (P.last_kind == 1 && !lab) {
+    // u2's set is N(x): per-lane binary searches
+    for (uint32_t base = cbeg; base < cend; base += kWave) {
+      const uint32_t i = base + lane;
+      if (i < cend) {
+        const uint32_t x = cand[i];
+        if (admit(g, P, ws, u1, P.cand_check[d], x)) {
+          const uint32_t* nx = g.colidx + g.rowptr[x];
+          const uint32_t dx = degree(g, x);
+          const uint32_t ub = sym_x ? min(ub_fixed, x) : ub_fixed;
+          const uint32_t c = below(nx, dx, ub);
+          uint32_t dr = 0, b = dropm;
+          while (b) {
+            int j = __builtin_ctz(b);
+            b &= b - 1;
+            const uint32_t y = ws.phi[j];
+            dr += (y < ub) && member(nx, c, y);
+          }
+          acc += c - dr;
+          ct.alg += 20ull;
+        }
+      }
+    }
+    ct.nops++;
+    return wave_sum(acc);
+  }
+
+  // Edge-parallel pass over the candidates' neighbour lists.  kind 1
+  // (labeled): every y in N(x) counts; kind 2: y must lie in the parent set
+  // P (LDS hash when it fits, else binary search in P).
+  const bool isect = P.last_kind == 2;
+  const uint32_t* Pset = isect ? uniptr(ws.sptr[P.last_src]) : nullptr;
+  uint32_t nP = isect ? uni(ws.slen[P.last_src]) : 0;
+  if (isect && ub_fixed != kInf) nP = uni(below(Pset, nP, ub_fixed));
+  const bool use_hash = isect && nP <= (uint32_t)(kHashCap / 2);
+  if (use_hash) {
+    for (int i = lane; i < kHashCap; i += kWave) ws.hash[i] = kEmpty;
+    __syncthreads();
+    for (uint32_t i = lane; i < nP; i += kWave) {
+      const uint32_t y = Pset[i];
+      uint32_t h = hslot(y);
+      while (atomicCAS(&ws.hash[h], kEmpty, y) != kEmpty) h = (h + 1) & (kHashCap - 1);
+    }
+    __syncthreads();
+    ct.alg += 4ull * nP;
+  }
+  uint32_t inP = 0;  // bit j: phi(j) in P (unlabeled injectivity)
+  if (isect && !lab) {
+    bool in = false;
+    if (lane <= (uint32_t)d && ((dropm >> lane) & 1u)) in = member(Pset, nP, ws.phi[lane]);
+    inP = (uint32_t)__ballot(in);
+  }
+  const bool check2 = lab && P.last_check;
+  const uint64_t hub_lim = isect ? (uint64_t)P.hub_factor * (uint64_t)max(nP, 1u) : ~0ull;
+  for (uint32_t base = cbeg; base < cend; base += kWave) {
+    const uint32_t i = base + lane;
+    uint32_t x = 0, dx = 0, ub = ub_fixed;
+    unsigned long long wx = 0;
+    uint64_t rp = 0;
+    bool ok = false, hub = false;
+    if (i < cend) {
+      x = cand[i];
+      ok = admit(g, P, ws, u1, P.cand_check[d], x);
+      if (ok) {
+        rp = g.rowptr[x];
+        dx = (uint32_t)(g.rowptr[x + 1] - rp);
+        wx = wd * (w1 ? w1[x] : 1ull);
+        if (sym_x) ub = min(ub, x);
+        if (!lab) {
+          uint32_t b = inP;
+          const uint32_t* nx = g.colidx + rp;
+          while (b) {
+            int j = __builtin_ctz(b);
+            b &= b - 1;
+            const uint32_t y = ws.phi[j];
+            dropc += (y < ub) && member(nx, dx, y);
+          }
+        }
+        hub = (uint64_t)dx > hub_lim;
+      }
+    }
+    // exclusive scan of the owners' list lengths
+    const uint32_t len = (ok && !hub) ? dx : 0;
+    uint32_t incl = len;
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)incl, off);
+      if ((int)lane >= off) incl += t;
+    }
+    ws.own_off[lane] = incl - len;
+    if (lane == kWave - 1) ws.own_off[kWave] = incl;
+    ws.own_ub[lane] = ub;
+    ws.own_w[lane] = (ok && !hub) ? wx : 0ull;
+    ws.own_rp[lane] = rp;
+    __syncthreads();
+    const uint32_t total = uni(ws.own_off[kWave]);
+    ct.alg += 4ull * total + 16ull * kWave;
+    for (uint32_t t = lane; t < total; t += kWave) {
+      // owner: last o with own_off[o] <= t
+      uint32_t lo = 0, hi = kWave;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ws.own_off[mid] <= t) lo = mid; else hi = mid;
+      }
+      const uint32_t y = g.colidx[ws.own_rp[lo] + (t - ws.own_off[lo])];
+      if (y >= ws.own_ub[lo]) continue;
+      bool hit = true;
+      if (isect) {
+        if (use_hash) {
+          uint32_t h = hslot(y);
+          uint32_t e;
+          while ((e = ws.hash[h]) != kEmpty && e != y) h = (h + 1) & (kHashCap - 1);
+          hit = e == y;
+        } else {
+          hit = member(Pset, nP, y);
+        }
+      }
+      if (hit && check2) hit = g.lab[y] == P.lab[u2];
+      if (hit) acc += ws.own_w[lo] * (w2 ? w2[y] : 1ull);
+    }
+    __syncthreads();
+    // owners whose list is far longer than P: one by one, iterating P and
+    // searching N(x)
+    uint64_t hubs = __ballot(hub);
+    while (hubs) {
+      const int o = __builtin_ctzll(hubs);
+      hubs &= hubs - 1;
+      const uint32_t xo = uni((uint32_t)__shfl((int)x, o));
+      const uint32_t ubo = uni((uint32_t)__shfl((int)ub, o));
+      const unsigned long long wo = uni64(__shfl(wx, o));
+      const uint32_t* nx = uniptr(g.colidx + g.rowptr[xo]);
+      uint32_t nxn = uni(degree(g, xo));
+      if (ubo != kInf) nxn = uni(below(nx, nxn, ubo));
+      const uint32_t nPo = ubo != kInf ? uni(below(Pset, nP, ubo)) : nP;
+      unsigned long long s = 0;
+      uint32_t c;
+      if (lab)
+        c = wave_isect<false, true, true>(Pset, nPo, nx, nxn,
+                                          P.last_check ? (1u << P.lab[u2]) : 0xFFFFFFFFu, g.lab,
+                                          w2, nullptr, lane, ct, &s);
+      else
+        c = wave_isect<false, false, false>(Pset, nPo, nx, nxn, 0, g.lab, nullptr, nullptr, lane,
+                                            ct, nullptr);
+      if (lane == 0) acc += wo * (lab && w2 ? s : (unsigned long long)c);
+    }
+  }
+  ct.nops++;
+  return wave_sum(acc) - wave_sum(dropc);
+}
+
+// DFS over one job (edge e = (r,c)) restricted to vertex-2 candidates
+// [lo, hi).  Phase 1 (may_split) may hand the tail of a long range to the
+// task buffer.
+__device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState& ws,
+                                      uint32_t* arena, uint32_t cap, uint32_t lane, uint32_t r,
+                                      uint32_t c, uint64_t e, uint32_t lo, uint32_t hi,
+                                      bool may_split, Task* tasks, unsigned long long* ntasks,
+                                      uint64_t task_cap, Counters& ct) {
+  const int k = P.k;
+  ws.phi[0] = r;
+  ws.phi[1] = c;
+  const unsigned long long w01 = uni64(wt_of(P, 0, r) * wt_of(P, 1, c));
+  ws.wacc[1] = w01;
+  if (k == 2) return w01;
+  run_ops(g, P, ws, 0, arena, cap, lane, ct);
+  if (k == 3) return w01 * last_count(g, P, ws, lane, ct);
+  run_ops(g, P, ws, 1, arena, cap, lane, ct);
+  setup_cand(P, ws, 1);
+  const uint32_t n2 = uni(ws.clen[1]);
+  const uint32_t b = min(lo, n2);
+  uint32_t en = min(hi, n2);
+  const uint32_t G = P.split;
+  if (may_split && G > 0 && en > b + G) {
+    // blocks t = 0..want-1 cover [b+G+t*G, ...); the last `fit` go to tasks
+    const uint32_t keep_end = b + G;
+    const uint64_t want = (en - keep_end + G - 1) / G;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(ntasks, (unsigned long long)want);
+    base = uni64(__shfl(base, 0));
+    const uint64_t fit = base >= task_cap ? 0 : min(want, task_cap - base);
+    const uint64_t first = want - fit;
+    for (uint64_t t = first + lane; t < want; t += kWave) {
+      Task tk;
+      tk.edge = e;
+      tk.lo = (uint32_t)(keep_end + t * G);
+      tk.hi = (uint32_t)min((uint64_t)keep_end + (t + 1) * G, (uint64_t)en);
+      tasks[base + (t - first)] = tk;
+    }
+    en = (uint32_t)min((uint64_t)en, (uint64_t)keep_end + first * G);
+  }
+  ws.cnext[1] = b;
+  ws.clen[1] = en;
+  if (k == 4) return fused_tail(g, P, ws, 1, lane, ct);
+  unsigned long long cnt = 0;
+  int d = 1;
+  for (;;) {
+    uint32_t x;
+    if (!next_cand(g, P, ws, d, lane, x, ct)) {
+      if (d == 1) break;
+      d--;
+      continue;
+    }
+    ws.phi[d + 1] = x;
+    ws.wacc[d + 1] = uni64(ws.wacc[d]) * wt_of(P, d + 1, x);
+    run_ops(g, P, ws, d + 1, arena, cap, lane, ct);
+    setup_cand(P, ws, d + 1);
+    d++;
+    if (d == k - 3) {
+      cnt += fused_tail(g, P, ws, d, lane, ct);
+      d--;
+    }
+  }
   return cnt;
 }
 
@@ -335,7 +681,6 @@ __device__ __forceinline__ uint32_t source_of(const KGraph& g, uint64_t e, uint3
     lo += step;
     step <<= 1;
   }
-  // now rowptr[lo] <= e < rowptr[min(lo+step, n)]
   uint32_t hi = min(lo + step, n);
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
@@ -344,36 +689,47 @@ __device__ __forceinline__ uint32_t source_of(const KGraph& g, uint64_t e, uint3
   return lo;
 }
 
-__global__ __launch_bounds__(64) void k_match(KGraph g, const KPlan* __restrict__ Pp,
-                                              unsigned long long* cursor, uint64_t edge_lo,
-                                              uint64_t edge_hi, uint32_t chunk, uint32_t* arena,
-                                              uint64_t arena_stride, uint32_t cap,
-                                              unsigned long long* out, unsigned long long* per_edge) {
-  __shared__ WaveState ws;
-  const KPlan& P = *Pp;
-  const uint32_t lane = threadIdx.x;
-  uint32_t* my_arena = arena + (size_t)blockIdx.x * arena_stride;
-  uint64_t total = 0, jobs = 0, alg = 0, nops = 0;
-  const uint64_t span = edge_hi - edge_lo;
-  const int k = P.k;
+__device__ __forceinline__ uint32_t source_full(const KGraph& g, uint64_t e) {
+  uint32_t lo = 0, hi = (uint32_t)g.n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (g.rowptr[mid] <= e) lo = mid; else hi = mid;
+  }
+  return lo;
+}
 
+struct Launch {
+  KGraph g;
+  const KPlan* plan;
+  unsigned long long* cursor;  // [0] edge cursor, [1] task cursor, [2] task count
+  uint64_t edge_lo, edge_hi;
+  uint32_t chunk;
+  uint32_t cap;
+  uint32_t* arena;
+  uint64_t arena_stride;
+  unsigned long long* out;       // count, jobs, alg bytes, set ops
+  unsigned long long* per_edge;  // optional, indexed e - edge_lo
+  Task* tasks;
+  uint64_t task_cap;
+};
+
+__global__ __launch_bounds__(64) void k_match_edges(Launch L) {
+  __shared__ WaveState ws;
+  const KGraph& g = L.g;
+  const KPlan& P = *L.plan;
+  const uint32_t lane = threadIdx.x;
+  uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
+  unsigned long long total = 0, jobs = 0;
+  Counters ct;
+  const uint64_t span = L.edge_hi - L.edge_lo;
   for (;;) {
     unsigned long long c0 = 0;
-    if (lane == 0) c0 = atomicAdd(cursor, (unsigned long long)chunk);
+    if (lane == 0) c0 = atomicAdd(&L.cursor[0], (unsigned long long)L.chunk);
     c0 = uni64(__shfl(c0, 0));
     if (c0 >= span) break;
-    const uint64_t e0 = edge_lo + c0;
-    const uint64_t e1 = min(e0 + chunk, edge_hi);
-    // source of e0: binary search over all vertices (uniform)
-    uint32_t rlo = 0;
-    {
-      uint32_t lo = 0, hi = (uint32_t)g.n;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (g.rowptr[mid] <= e0) lo = mid; else hi = mid;
-      }
-      rlo = uni(lo);
-    }
+    const uint64_t e0 = L.edge_lo + c0;
+    const uint64_t e1 = min(e0 + L.chunk, L.edge_hi);
+    uint32_t rlo = uni(source_full(g, e0));
     for (uint64_t eb = e0; eb < e1; eb += kWave) {
       const uint64_t e = eb + lane;
       bool ok = e < e1;
@@ -391,50 +747,124 @@ __global__ __launch_bounds__(64) void k_match(KGraph g, const KPlan* __restrict_
       ws.jc[lane] = c;
       __syncthreads();
       rlo = uni((uint32_t)__shfl((int)r, (int)lastl));
-      alg += 24ull * min((uint64_t)kWave, e1 - eb);
+      ct.alg += 24ull * min((uint64_t)kWave, e1 - eb);
       jm = uni64(jm);
       while (jm) {
         const int i = __builtin_ctzll(jm);
         jm &= jm - 1;
         jobs++;
-        ws.phi[0] = uni(ws.jr[i]);
-        ws.phi[1] = uni(ws.jc[i]);
-        uint64_t cnt = 0;
-        run_ops(g, P, ws, 0, my_arena, cap, lane, alg, nops);
-        if (k == 3) {
-          cnt = last_count(g, P, ws, lane, alg, nops);
-        } else {
-          run_ops(g, P, ws, 1, my_arena, cap, lane, alg, nops);
-          setup_cand(P, ws, 1);
-          int d = 1;
-          for (;;) {
-            uint32_t x;
-            if (!next_cand(g, P, ws, d, lane, x, alg)) {
-              if (d == 1) break;
-              d--;
-              continue;
-            }
-            ws.phi[d + 1] = x;
-            if (d + 1 == k - 2) {
-              cnt += last_count(g, P, ws, lane, alg, nops);
-              continue;
-            }
-            run_ops(g, P, ws, d + 1, my_arena, cap, lane, alg, nops);
-            setup_cand(P, ws, d + 1);
-            d++;
-          }
-        }
+        const uint32_t jr = uni(ws.jr[i]), jc = uni(ws.jc[i]);
+        const unsigned long long cnt =
+            run_job(g, P, ws, arena, L.cap, lane, jr, jc, eb + i, 0, kInf, L.tasks != nullptr,
+                    L.tasks, &L.cursor[2], L.task_cap, ct);
         total += cnt;
-        if (per_edge && lane == 0) per_edge[eb + i - edge_lo] = cnt;
+        if (L.per_edge && lane == 0) atomicAdd(&L.per_edge[eb + i - L.edge_lo], cnt);
       }
       __syncthreads();
     }
   }
   if (lane == 0) {
-    out[blockIdx.x * 4 + 0] = total;
-    out[blockIdx.x * 4 + 1] = jobs;
-    out[blockIdx.x * 4 + 2] = alg;
-    out[blockIdx.x * 4 + 3] = nops;
+    atomicAdd(&L.out[0], total);
+    atomicAdd(&L.out[1], jobs);
+    atomicAdd(&L.out[2], (unsigned long long)ct.alg);
+    atomicAdd(&L.out[3], (unsigned long long)ct.nops);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_match_tasks(Launch L) {
+  __shared__ WaveState ws;
+  const KGraph& g = L.g;
+  const KPlan& P = *L.plan;
+  const uint32_t lane = threadIdx.x;
+  uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
+  unsigned long long total = 0;
+  Counters ct;
+  const uint64_t ntask = min((uint64_t)L.cursor[2], L.task_cap);
+  for (;;) {
+    unsigned long long t = 0;
+    if (lane == 0) t = atomicAdd(&L.cursor[1], 1ull);
+    t = uni64(__shfl(t, 0));
+    if (t >= ntask) break;
+    const uint64_t e = uni64(L.tasks[t].edge);
+    const uint32_t tlo = uni(L.tasks[t].lo), thi = uni(L.tasks[t].hi);
+    const uint32_t r = uni(source_full(g, e));
+    const uint32_t c = uni(g.colidx[e]);
+    const unsigned long long cnt =
+        run_job(g, P, ws, arena, L.cap, lane, r, c, e, tlo, thi, false, nullptr, nullptr, 0, ct);
+    total += cnt;
+    if (L.per_edge && lane == 0) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
+  }
+  if (lane == 0) {
+    atomicAdd(&L.out[0], total);
+    atomicAdd(&L.out[4], (unsigned long long)ct.alg);
+    atomicAdd(&L.out[5], (unsigned long long)ct.nops);
+  }
+}
+
+// F(v) = sum over x in N(v) with lab[x] == want of prod_c child_c[x]: one
+// wave per group of 64 consecutive rows, walking the group's edges 64 at a
+// time (coalesced colidx reads); each lane finds its row in the LDS copy of
+// the group's row offsets and adds into an LDS accumulator.
+struct FoldArgs {
+  KGraph g;
+  uint32_t want;
+  int nchild;
+  const unsigned long long* child[kMaxK];
+  unsigned long long* out;
+  unsigned long long* bytes;  // algorithmic bytes accumulator
+};
+
+__global__ __launch_bounds__(64) void k_fold(FoldArgs A) {
+  __shared__ uint64_t rp[kWave + 1];
+  __shared__ unsigned long long acc[kWave];
+  const KGraph& g = A.g;
+  const uint32_t lane = threadIdx.x;
+  const uint64_t ngroups = (g.n + kWave - 1) / kWave;
+  unsigned long long hits = 0, edges = 0, rows = 0;
+  for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const uint64_t v0 = grp * kWave;
+    const uint32_t nv = (uint32_t)min((uint64_t)kWave, g.n - v0);
+    rp[lane] = g.rowptr[v0 + min(lane, nv)];
+    if (lane == 0) rp[kWave] = g.rowptr[v0 + nv];
+    acc[lane] = 0;
+    __syncthreads();
+    const uint64_t e0 = rp[0], e1 = rp[kWave];
+    for (uint64_t eb = e0; eb < e1; eb += kWave) {
+      const uint64_t e = eb + lane;
+      if (e < e1) {
+        uint32_t lo = 0, hi = kWave;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (rp[mid] <= e) lo = mid; else hi = mid;
+        }
+        const uint32_t x = g.colidx[e];
+        if (g.lab[x] == A.want) {
+          unsigned long long p = 1;
+          for (int c = 0; c < A.nchild; c++) p *= A.child[c][x];
+          atomicAdd(&acc[lo], p);
+          hits++;
+        }
+      }
+    }
+    __syncthreads();
+    if (lane < nv) A.out[v0 + lane] = acc[lane];
+    edges += e1 - e0;
+    rows += nv;
+    __syncthreads();
+  }
+  // colidx 4 B + label 1 B per edge, 8 B per child value gathered, row
+  // offsets 8 B and the result 8 B per row
+  hits = wave_sum(hits);
+  if (lane == 0) atomicAdd(A.bytes, 5ull * edges + 8ull * A.nchild * hits + 16ull * rows);
+}
+
+// out[v] = prod_c child_c[v]
+__global__ void k_product(uint64_t n, FoldArgs A) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long p = 1;
+    for (int c = 0; c < A.nchild; c++) p *= A.child[c][v];
+    A.out[v] = p;
   }
 }
 
@@ -444,13 +874,13 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     p[i] = v;
 }
 
-#define HIP_OK(call)                                                              \
-  do {                                                                            \
-    hipError_t _e = (call);                                                       \
-    if (_e != hipSuccess) {                                                       \
-      set_error(std::string(#call) + ": " + hipGetErrorString(_e));               \
-      return PM_ERR_DEVICE;                                                       \
-    }                                                                             \
+#define HIP_OK(call)                                                \
+  do {                                                              \
+    hipError_t _e = (call);                                         \
+    if (_e != hipSuccess) {                                         \
+      set_error(std::string(#call) + ": " + hipGetErrorString(_e)); \
+      return PM_ERR_DEVICE;                                         \
+    }                                                               \
   } while (0)
 
 // Per-process work areas, grown on demand and reused across calls.
@@ -459,11 +889,14 @@ struct Workspace {
   hipStream_t stream = nullptr;
   void* arena = nullptr;
   size_t arena_bytes = 0;
-  unsigned long long* cursor = nullptr;
-  unsigned long long* out = nullptr;
-  size_t out_waves = 0;
+  unsigned long long* cursor = nullptr;  // 8 words
+  unsigned long long* out = nullptr;     // 8 words
   KPlan* plan = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Task* tasks = nullptr;
+  uint64_t task_cap = 0;
+  void* tables = nullptr;  // fold / weight tables
+  size_t table_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, eve = nullptr;
   int cus = 0;
 };
 std::mutex g_ws_mu;
@@ -473,18 +906,34 @@ int ensure_workspace(int device) {
   if (g_ws.device == device) return PM_OK;
   HIP_OK(hipSetDevice(device));
   HIP_OK(hipStreamCreateWithFlags(&g_ws.stream, hipStreamNonBlocking));
-  HIP_OK(hipMalloc(&g_ws.cursor, 64));
+  HIP_OK(hipMalloc(&g_ws.cursor, 8 * sizeof(unsigned long long)));
+  HIP_OK(hipMalloc(&g_ws.out, 8 * sizeof(unsigned long long)));
   HIP_OK(hipMalloc(&g_ws.plan, sizeof(KPlan)));
   HIP_OK(hipEventCreate(&g_ws.ev0));
   HIP_OK(hipEventCreate(&g_ws.ev1));
+  HIP_OK(hipEventCreate(&g_ws.evf));
+  HIP_OK(hipEventCreate(&g_ws.eve));
   HIP_OK(hipDeviceGetAttribute(&g_ws.cus, hipDeviceAttributeMultiprocessorCount, device));
+  g_ws.task_cap = 1ull << 22;  // 4M range tasks (64 MiB)
+  HIP_OK(hipMalloc(&g_ws.tasks, sizeof(Task) * g_ws.task_cap));
   g_ws.device = device;
+  return PM_OK;
+}
+
+int grow(void** p, size_t* have, size_t need) {
+  if (need <= *have) return PM_OK;
+  if (*p) HIP_OK(hipFree(*p));
+  *p = nullptr;
+  *have = 0;
+  HIP_OK(hipMalloc(p, need));
+  *have = need;
   return PM_OK;
 }
 
 void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
   std::memset(kp, 0, sizeof(*kp));
-  kp->k = p.k;
+  const int kc = p.kc;
+  kp->k = kc;
   kp->labeled = p.labeled;
   kp->rc_order = p.rc_order;
   kp->deg0 = p.deg[0];
@@ -492,16 +941,16 @@ void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
   kp->deg2 = p.deg[2];
   kp->lab0 = p.vlabel[0];
   kp->lab1 = p.vlabel[1];
-  for (int u = 0; u < p.k; u++) {
-    kp->lab[u] = p.vlabel[u];
-    kp->B[u] = p.B[u];
-    kp->X[u] = p.X[u];
+  for (int u = 0; u < kc; u++) {
+    kp->lab[u] = p.clabel[u];
+    kp->B[u] = p.cB[u];
+    kp->X[u] = p.cX[u];
   }
   for (int s = 0; s < kMaxSlots; s++) kp->region[s] = -1;
   int nops = 0, nreg = 0;
-  for (int d = 0; d <= p.k - 2; d++) {
+  for (int d = 0; d <= kc - 2; d++) {
     kp->op_begin[d] = (int8_t)nops;
-    if (d < p.k - 2) {
+    if (d < kc - 2) {
       for (const SlotOp& op : p.ops[d]) {
         KOp& ko = kp->ops[nops++];
         ko.kind = op.kind;
@@ -516,32 +965,29 @@ void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
     kp->op_end[d] = (int8_t)nops;
   }
   kp->nops = nops;
-  for (int d = 1; d <= p.k - 3; d++) {
+  auto exact_slot = [&](int s, int label, int upto) {
+    for (int dd = 0; dd <= upto; dd++)
+      for (const SlotOp& op : p.ops[dd])
+        if (op.slot == s && op.kind == 1 && op.labmask == (1u << label)) return true;
+    return false;
+  };
+  for (int d = 1; d <= kc - 3; d++) {
     const int s = p.slot_of[d][d + 1];
     kp->cand_slot[d] = (int8_t)s;
-    // a materialised slot whose label filter is exactly this vertex's label
-    // needs no further test
-    bool exact = false;
-    for (int dd = 0; dd <= d; dd++)
-      for (const SlotOp& op : p.ops[dd])
-        if (op.slot == s && op.kind == 1 && op.labmask == (1u << p.vlabel[d + 1])) exact = true;
-    kp->cand_check[d] = exact ? 0 : 1;
+    kp->cand_check[d] = exact_slot(s, p.clabel[d + 1], d) ? 0 : 1;
   }
-  const int dl = p.k - 2;
-  if (p.ops[dl].empty()) {
-    kp->last_kind = 0;
-    kp->last_src = (int8_t)p.slot_of[dl][p.k - 1];
-    bool exact = false;
-    for (int dd = 0; dd < dl; dd++)
-      for (const SlotOp& op : p.ops[dd])
-        if (op.slot == kp->last_src && op.kind == 1 && op.labmask == (1u << p.vlabel[p.k - 1]))
-          exact = true;
-    kp->last_check = exact ? 0 : 1;
-  } else {
-    const SlotOp& op = p.ops[dl][0];
-    kp->last_kind = op.kind == 0 ? 1 : 2;
-    kp->last_src = op.parent;
-    kp->last_check = 1;
+  if (kc >= 3) {
+    const int dl = kc - 2;
+    if (p.ops[dl].empty()) {
+      kp->last_kind = 0;
+      kp->last_src = (int8_t)p.slot_of[dl][kc - 1];
+      kp->last_check = exact_slot(kp->last_src, p.clabel[kc - 1], dl - 1) ? 0 : 1;
+    } else {
+      const SlotOp& op = p.ops[dl][0];
+      kp->last_kind = op.kind == 0 ? 1 : 2;
+      kp->last_src = op.parent;
+      kp->last_check = (op.kind == 1 && exact_slot(op.parent, p.clabel[kc - 1], dl - 1)) ? 0 : 1;
+    }
   }
   *nregions = nreg;
 }
@@ -603,85 +1049,141 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   rc = ensure_workspace(g->device);
   if (rc) return rc;
   const HostGraph& h = g->h;
+  if (plan.labeled && h.labels.empty()) {
+    set_error("labeled matching needs vertex labels (open the graph with PM_GRAPH_LABELS)");
+    return PM_ERR_ARG;
+  }
   KPlan kp;
   int nreg = 0;
   build_kplan(plan, &kp, &nreg);
-  if (plan.labeled) {
-    if (h.labels.empty()) {
-      set_error("labeled matching needs vertex labels (open the graph with PM_GRAPH_LABELS)");
-      return PM_ERR_ARG;
-    }
-  }
   hipStream_t st = opts->stream ? (hipStream_t)opts->stream : g_ws.stream;
-  int waves = opts->waves > 0 ? opts->waves : g_ws.cus * 16;
-  uint32_t chunk = opts->chunk > 0 ? (uint32_t)opts->chunk : 64u;
-  uint64_t lo = opts->edge_lo, hi = opts->edge_hi ? opts->edge_hi : h.m;
+  const int waves = opts->waves > 0 ? opts->waves : g_ws.cus * 16;
+  const uint64_t lo = opts->edge_lo, hi = opts->edge_hi ? opts->edge_hi : h.m;
   if (hi > h.m || lo > hi) {
     set_error("edge range outside the graph");
     return PM_ERR_ARG;
   }
+  const uint32_t chunk =
+      opts->chunk > 0 ? (uint32_t)opts->chunk
+                      : (uint32_t)std::max<uint64_t>(
+                            1, std::min<uint64_t>(64, (hi - lo) / ((uint64_t)waves * 8)));
+  kp.split = opts->split > 0 ? (uint32_t)opts->split : 64u;
+  kp.hub_factor = opts->hub_factor > 0 ? (uint32_t)opts->hub_factor : 8u;
+
+  // fold / weight tables: one u64[n] per folded vertex and per weighted
+  // core vertex
+  const uint64_t nn = std::max<uint64_t>(h.n, 1);
+  int table_of[kMaxK];
+  int ntab = 0;
+  for (int u = 0; u < kMaxK; u++) table_of[u] = -1;
+  for (int i = 0; i < plan.nfold; i++) table_of[plan.fold_v[i]] = ntab++;
+  int core_tab[kMaxK];
+  for (int i = 0; i < plan.kc; i++) core_tab[i] = plan.fold_children[plan.core_v[i]] ? ntab++ : -1;
+  if (ntab) {
+    rc = grow(&g_ws.tables, &g_ws.table_bytes, sizeof(unsigned long long) * ntab * nn);
+    if (rc) return rc;
+  }
+  auto table = [&](int t) { return (unsigned long long*)g_ws.tables + (size_t)t * nn; };
+  for (int i = 0; i < plan.kc; i++) kp.wt[i] = core_tab[i] >= 0 ? table(core_tab[i]) : nullptr;
+
   const uint32_t cap = std::max<uint32_t>(h.max_degree, 1);
   const uint64_t stride = (uint64_t)std::max(nreg, 1) * cap;
-  const size_t need = (size_t)waves * stride * sizeof(uint32_t);
-  if (need > g_ws.arena_bytes) {
-    if (g_ws.arena) HIP_OK(hipFree(g_ws.arena));
-    g_ws.arena = nullptr;
-    g_ws.arena_bytes = 0;
-    HIP_OK(hipMalloc(&g_ws.arena, need));
-    g_ws.arena_bytes = need;
-  }
-  if ((size_t)waves > g_ws.out_waves) {
-    if (g_ws.out) HIP_OK(hipFree(g_ws.out));
-    HIP_OK(hipMalloc(&g_ws.out, sizeof(unsigned long long) * 4 * waves));
-    g_ws.out_waves = waves;
-  }
+  rc = grow(&g_ws.arena, &g_ws.arena_bytes, (size_t)waves * stride * sizeof(uint32_t));
+  if (rc) return rc;
   HIP_OK(hipMemcpyAsync(g_ws.plan, &kp, sizeof(kp), hipMemcpyHostToDevice, st));
   unsigned long long* d_per_edge = nullptr;
-  if (opts->per_edge && hi > lo) {
+  if (opts->per_edge && hi > lo)
     HIP_OK(hipMallocAsync((void**)&d_per_edge, sizeof(unsigned long long) * (hi - lo), st));
-    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, st, d_per_edge, hi - lo, 0ull);
-  }
-  KGraph kg{h.n, h.m, g->d.rowptr, g->d.colidx, g->d.label8, g->d.maxnbr};
+  const KGraph kg{h.n, h.m, g->d.rowptr, g->d.colidx, g->d.label8, g->d.maxnbr};
+  Launch L;
+  L.g = kg;
+  L.plan = g_ws.plan;
+  L.cursor = g_ws.cursor;
+  L.edge_lo = lo;
+  L.edge_hi = hi;
+  L.chunk = chunk;
+  L.cap = cap;
+  L.arena = (uint32_t*)g_ws.arena;
+  L.arena_stride = stride;
+  L.out = g_ws.out;
+  L.tasks = (plan.kc >= 4 && opts->split >= 0) ? g_ws.tasks : nullptr;
+  L.task_cap = g_ws.task_cap;
   const int reps = opts->repeats > 0 ? opts->repeats : 1;
-  double sum_ms = 0, min_ms = 1e30;
-  std::vector<unsigned long long> out(4 * (size_t)waves);
+  double sum_ms = 0, min_ms = 1e30, phase[3] = {0, 0, 0};
+  const int fold_blocks =
+      (int)std::max<uint64_t>(1, std::min<uint64_t>((h.n + kWave - 1) / kWave, (uint64_t)g_ws.cus * 32));
   for (int r = 0; r < reps; r++) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor, 1, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor, 8, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.out, 8, 0ull);
+    L.per_edge = (r == 0) ? d_per_edge : nullptr;
+    if (L.per_edge)
+      hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, st, d_per_edge, hi - lo, 0ull);
     HIP_OK(hipEventRecord(g_ws.ev0, st));
-    hipLaunchKernelGGL(k_match, dim3(waves), dim3(kWave), 0, st, kg, g_ws.plan, g_ws.cursor, lo, hi,
-                       chunk, (uint32_t*)g_ws.arena, stride, cap, g_ws.out,
-                       (r == 0) ? d_per_edge : nullptr);
+    // pendant-tree tables, children first
+    for (int i = 0; i < plan.nfold; i++) {
+      const int u = plan.fold_v[i];
+      FoldArgs A;
+      std::memset(&A, 0, sizeof(A));
+      A.g = kg;
+      A.want = (uint32_t)plan.vlabel[u];
+      for (int c = 0; c < plan.k; c++)
+        if (plan.fold_children[u] >> c & 1u) A.child[A.nchild++] = table(table_of[c]);
+      A.out = table(table_of[u]);
+      A.bytes = g_ws.out + 6;
+      if (h.n) hipLaunchKernelGGL(k_fold, dim3(fold_blocks), dim3(kWave), 0, st, A);
+    }
+    for (int i = 0; i < plan.kc; i++) {
+      if (core_tab[i] < 0) continue;
+      FoldArgs A;
+      std::memset(&A, 0, sizeof(A));
+      const int u = plan.core_v[i];
+      for (int c = 0; c < plan.k; c++)
+        if (plan.fold_children[u] >> c & 1u) A.child[A.nchild++] = table(table_of[c]);
+      A.out = table(core_tab[i]);
+      A.bytes = g_ws.out + 6;
+      if (h.n) hipLaunchKernelGGL(k_product, dim3(1024), dim3(256), 0, st, h.n, A);
+    }
+    HIP_OK(hipEventRecord(g_ws.evf, st));
+    hipLaunchKernelGGL(k_match_edges, dim3(waves), dim3(kWave), 0, st, L);
+    HIP_OK(hipEventRecord(g_ws.eve, st));
+    if (L.tasks) hipLaunchKernelGGL(k_match_tasks, dim3(waves), dim3(kWave), 0, st, L);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(g_ws.ev1, st));
     HIP_OK(hipEventSynchronize(g_ws.ev1));
-    float ms = 0;
+    float ms = 0, mf = 0, me = 0, mt = 0;
     HIP_OK(hipEventElapsedTime(&ms, g_ws.ev0, g_ws.ev1));
+    HIP_OK(hipEventElapsedTime(&mf, g_ws.ev0, g_ws.evf));
+    HIP_OK(hipEventElapsedTime(&me, g_ws.evf, g_ws.eve));
+    HIP_OK(hipEventElapsedTime(&mt, g_ws.eve, g_ws.ev1));
+    phase[0] = mf;
+    phase[1] = me;
+    phase[2] = mt;
     sum_ms += ms;
     min_ms = std::min(min_ms, (double)ms);
   }
-  HIP_OK(hipMemcpyAsync(out.data(), g_ws.out, sizeof(unsigned long long) * 4 * waves,
-                        hipMemcpyDeviceToHost, st));
+  unsigned long long out[8], cur[8];
+  HIP_OK(hipMemcpyAsync(out, g_ws.out, sizeof(out), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(cur, g_ws.cursor, sizeof(cur), hipMemcpyDeviceToHost, st));
   if (d_per_edge) {
     HIP_OK(hipMemcpyAsync(opts->per_edge + lo, d_per_edge, sizeof(unsigned long long) * (hi - lo),
                           hipMemcpyDeviceToHost, st));
     HIP_OK(hipFreeAsync(d_per_edge, st));
   }
   HIP_OK(hipStreamSynchronize(st));
-  uint64_t total = 0, jobs = 0, alg = 0, nops = 0;
-  for (int w = 0; w < waves; w++) {
-    total += out[4 * w];
-    jobs += out[4 * w + 1];
-    alg += out[4 * w + 2];
-    nops += out[4 * w + 3];
-  }
+  uint64_t total = out[0];
   if (!plan.labeled) total *= (uint64_t)plan.aut;  // test.cu:198
   std::memset(res, 0, sizeof(*res));
   res->kernel_ms = sum_ms / reps;
   res->min_ms = min_ms;
   res->count = total;
-  res->jobs = jobs;
-  res->alg_bytes = alg;
-  res->set_ops = nops;
+  res->jobs = out[1];
+  res->alg_bytes = out[2] + out[4] + out[6];
+  res->set_ops = out[3] + out[5];
+  for (int i = 0; i < 3; i++) res->phase_ms[i] = phase[i];
+  res->phase_alg_bytes[0] = out[6];
+  res->phase_alg_bytes[1] = out[2];
+  res->phase_alg_bytes[2] = out[4];
+  res->tasks = std::min<uint64_t>(cur[2], g_ws.task_cap);
   res->waves = waves;
   res->chunk = (int32_t)chunk;
   return PM_OK;

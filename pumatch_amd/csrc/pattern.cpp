@@ -162,26 +162,71 @@ int build_plan(Plan* pp, std::string* err) {
   }
   p.rc_order = (!p.labeled && (p.B[1] & 1u)) ? 1 : 0;
 
-  // ---- slots ----
+  // ---- folding of pendant trees (homomorphism semantics only) ----
+  bool alive[kMaxK];
+  for (int u = 0; u < k; u++) alive[u] = true;
+  p.nfold = 0;
+  for (int u = 0; u < k; u++) p.fold_children[u] = 0;
+  if (p.labeled) {
+    bool changed = true;
+    while (changed) {
+      changed = false;
+      for (int u = k - 1; u >= 2; u--) {
+        if (!alive[u]) continue;
+        int nb = 0, par = -1;
+        for (int w = 0; w < k; w++)
+          if (w != u && alive[w] && p.adj[u][w] != 0) {
+            nb++;
+            par = w;
+          }
+        // every earlier neighbour of a live vertex is live, so the single
+        // live neighbour of a leaf is earlier than it
+        if (nb == 1 && par < u) {
+          alive[u] = false;
+          p.fold_v[p.nfold] = u;
+          p.fold_parent[p.nfold] = par;
+          p.nfold++;
+          p.fold_children[par] |= 1u << u;
+          changed = true;
+        }
+      }
+    }
+  }
+  p.kc = 0;
+  for (int u = 0; u < k; u++)
+    if (alive[u]) p.core_v[p.kc++] = u;
+  for (int i = 0; i < p.kc; i++) {
+    const int u = p.core_v[i];
+    p.clabel[i] = p.vlabel[u];
+    p.cA[i] = p.cX[i] = p.cB[i] = 0;
+    for (int j = 0; j < i; j++) {
+      const int w = p.core_v[j];
+      if (p.adj[u][w] != 0) p.cA[i] |= 1u << j;
+      else p.cX[i] |= 1u << j;
+      if (p.B[u] >> w & 1u) p.cB[i] |= 1u << j;
+    }
+  }
+
+  // ---- slots over the core ----
+  const int kc = p.kc;
   for (int d = 0; d < kMaxK; d++) {
     p.ops[d].clear();
     for (int u = 0; u < kMaxK; u++) p.slot_of[d][u] = -1;
   }
   p.nslots = 0;
   p.n_materialised = 0;
-  for (int d = 0; d <= k - 2; d++) {
+  for (int d = 0; d <= kc - 2; d++) {
     const uint32_t upto = (2u << d) - 1u;
-    for (int u = d + 1; u < k; u++) {
-      const uint32_t M = p.A[u] & upto;
+    for (int u = std::max(d + 1, 2); u < kc; u++) {
+      const uint32_t M = p.cA[u] & upto;
       if (M == 0) continue;
       if (!(M >> d & 1u)) {
         p.slot_of[d][u] = p.slot_of[d - 1][u];
         continue;
       }
-      // share with an earlier u of the same mask at this depth
       int found = -1;
       for (int w = d + 1; w < u; w++)
-        if ((p.A[w] & upto) == M && p.slot_of[d][w] >= 0) found = p.slot_of[d][w];
+        if ((p.cA[w] & upto) == M && p.slot_of[d][w] >= 0) found = p.slot_of[d][w];
       if (found >= 0) {
         p.slot_of[d][u] = found;
         continue;
@@ -199,20 +244,19 @@ int build_plan(Plan* pp, std::string* err) {
       } else {
         op.kind = 1;
         op.parent = (int8_t)p.slot_of[d - 1][u];
-        if (d < k - 2) p.n_materialised++;
+        if (d < kc - 2) p.n_materialised++;
       }
       op.labmask = 0;
       op.ncons = 0;
-      for (int w = d + 1; w < k; w++) {
-        if ((p.A[w] & upto) != M) continue;
-        op.labmask |= 1u << p.vlabel[w];
-        op.cons_b[op.ncons++] = (uint8_t)(p.B[w] & upto);
+      for (int w = std::max(d + 1, 2); w < kc; w++) {
+        if ((p.cA[w] & upto) != M) continue;
+        op.labmask |= 1u << p.clabel[w];
+        op.cons_b[op.ncons++] = (uint8_t)(p.cB[w] & upto);
       }
       p.slot_of[d][u] = op.slot;
       p.ops[d].push_back(op);
     }
   }
-  p.last_inline = !p.ops[k - 2].empty();
   return PM_OK;
 }
 

@@ -62,22 +62,40 @@ struct SlotOp {
 };
 
 struct Plan {
+  // ---- the pattern as read (file order = matching order) ----
   int k = 0;
   int labeled = 1;
   long long aut = 1;             // PatternMultiplicity, src/pattern.h:236
   int rc_order = 0;              // unlabeled job filter r > c
-  int deg[kMaxK] = {};           // pattern degree
+  int deg[kMaxK] = {};           // pattern degree (job filter uses deg[0..2])
   int vlabel[kMaxK] = {};        // pattern vertex label
   uint32_t A[kMaxK] = {};        // earlier neighbours
   uint32_t B[kMaxK] = {};        // symmetry pairs phi(u) < phi(j)
   uint32_t X[kMaxK] = {};        // earlier non-neighbours
   int adj[kMaxK][kMaxK] = {};    // edge labels as read
-  // execution plan
+
+  // ---- pendant-tree folding (labeled mode only) ----
+  // A vertex u >= 2 whose only unfolded neighbour is p < u is folded: its
+  // subtree contributes the factor F_u(phi(p)), with
+  //   F_u(v) = sum_{x in N(v), lab(x) = lab(u)} prod_{c child of u} F_c(x),
+  // exact for homomorphism counting.  fold_v lists folded vertices children
+  // first.
+  int nfold = 0;
+  int fold_v[kMaxK] = {};
+  int fold_parent[kMaxK] = {};
+  uint32_t fold_children[kMaxK] = {};  // by original id
+
+  // ---- the core that the DFS enumerates (core index -> original id) ----
+  int kc = 0;
+  int core_v[kMaxK] = {};
+  int clabel[kMaxK] = {};
+  uint32_t cA[kMaxK] = {}, cB[kMaxK] = {}, cX[kMaxK] = {};
+
+  // ---- execution plan over the core ----
   int nslots = 0;
   int slot_of[kMaxK][kMaxK];     // [depth][u] slot holding u's running set
-  std::vector<SlotOp> ops[kMaxK];  // ops run at depth d (d = 0..k-2)
+  std::vector<SlotOp> ops[kMaxK];  // ops run at depth d (d = 0..kc-2)
   int n_materialised = 0;        // INTERSECT slots (arena sizing)
-  int last_inline = 0;           // last vertex counted by an inline intersect
 };
 
 }  // namespace pm

@@ -1,0 +1,135 @@
+"""Host side of libpumatch.so: ABI exports, graph formats, generators,
+pattern parsing and planning.  No GPU needed."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import pumatch_amd as pm
+from tests.helpers import C1_EDGES, edges_to_adj, g_text
+
+
+def test_library_exports_every_declared_symbol(root):
+    header = open(os.path.join(root, "include", "pumatch.h")).read()
+    declared = set(re.findall(r"\b(pm_[a-z_]+)\s*\(", header))
+    assert declared, "no declarations found"
+    L = pm.lib()
+    for name in sorted(declared):
+        assert hasattr(L, name), f"{name} declared but not exported"
+    assert declared == set(pm._SIGS), "binding and header disagree"
+
+
+def test_rmat_is_deterministic_symmetric_sorted():
+    a = pm.Graph.rmat(10, 8, seed=3, device=False)
+    b = pm.Graph.rmat(10, 8, seed=3, device=False)
+    c = pm.Graph.rmat(10, 8, seed=4, device=False)
+    ra, ca, _ = a.csr()
+    rb, cb, _ = b.csr()
+    rc, cc, _ = c.csr()
+    assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+    assert not (ra.size == rc.size and ca.size == cc.size and np.array_equal(ca, cc))
+    n = ra.size - 1
+    src = np.repeat(np.arange(n), np.diff(ra).astype(np.int64))
+    assert np.all(src != ca), "self loop"
+    key = src.astype(np.int64) * n + ca
+    assert np.all(np.diff(key) > 0), "rows not strictly sorted"
+    fwd = set(zip(src.tolist(), ca.tolist()))
+    assert all((y, x) in fwd for x, y in fwd), "not symmetric"
+    assert a.info().max_degree == int(np.diff(ra).max())
+
+
+def test_chung_lu_shape():
+    g = pm.Graph.chung_lu(20000, 2.2, 500, 20, seed=4, device=False)
+    r, c, _ = g.csr()
+    deg = np.diff(r)
+    assert 10 < deg.mean() < 21
+    assert deg.max() <= 700 and deg[:10].mean() > 10 * deg.mean()
+
+
+def test_uniform_labels_range_and_seed():
+    g = pm.Graph.rmat(9, 8, seed=1, device=False)
+    g.set_uniform_labels(4, 2)
+    _, _, lab = g.csr()
+    assert lab.min() == 0 and lab.max() == 3
+    counts = np.bincount(lab, minlength=4)
+    assert counts.min() > 0.15 * lab.size
+
+
+def test_file_round_trip(tmp_path):
+    g = pm.Graph.rmat(9, 8, seed=2, device=False)
+    g.set_uniform_labels(3, 7)
+    prefix = str(tmp_path / "g")
+    g.write(prefix)
+    meta = open(prefix + ".meta.txt").read().split()
+    r, c, l = g.csr()
+    assert meta == [str(r.size - 1), str(c.size), "4", str(int(np.diff(r).max()))]
+    h = pm.Graph.open(prefix, labels=True, device=False)
+    r2, c2, l2 = h.csr()
+    assert np.array_equal(r, r2) and np.array_equal(c, c2) and np.array_equal(l, l2)
+    h2 = pm.Graph.open(prefix, labels=False, device=False)
+    assert h2.csr()[2] is None
+
+
+def test_open_missing_and_malformed(tmp_path):
+    with pytest.raises(pm.PumatchError):
+        pm.Graph.open(str(tmp_path / "nothing"), device=False)
+    (tmp_path / "bad.meta.txt").write_text("10 20 8 3\n")
+    with pytest.raises(pm.PumatchError):
+        pm.Graph.open(str(tmp_path / "bad"), labels=False, device=False)
+
+
+def test_from_csr_validation():
+    with pytest.raises(pm.PumatchError):  # unsorted adjacency
+        pm.Graph.from_csr([0, 2, 3, 4], [2, 1, 0, 0], device=False)
+    with pytest.raises(pm.PumatchError):  # label above 31
+        pm.Graph.from_csr([0, 1, 2], [1, 0], labels=[0, 40], device=False)
+    g = pm.Graph.from_csr([0, 1, 2], [1, 0], device=False)
+    assert (g.n, g.m) == (2, 2)
+
+
+def test_reference_patterns_plan_like_the_oracle(patterns_dir):
+    for i in range(1, 21):
+        path = os.path.join(patterns_dir, f"{i}.g")
+        text = open(path).read()
+        for labeled in (True, False):
+            info = pm.Pattern.load(path, labeled).info()
+            ref = oracle.plan_info(oracle.parse_g(text, labeled), labeled)
+            k = info.k
+            assert k == len(ref["deg"])
+            assert info.automorphisms == ref["aut"]
+            assert list(info.degree)[:k] == ref["deg"]
+            assert list(info.nbr_mask)[1:k] == ref["A"][1:]
+            if not labeled:
+                assert list(info.sym_mask)[:k] == ref["B"]
+                assert info.rc_order == ref["rc_order"]
+
+
+def test_pattern_errors():
+    with pytest.raises(pm.PumatchError):
+        pm.Pattern.parse("# t 0\n")
+    with pytest.raises(pm.PumatchError):  # 8 vertices
+        pm.Pattern.parse("\n".join(f"v {i} 1" for i in range(8)) + "\ne 0 1 1\n")
+    with pytest.raises(pm.PumatchError):  # endpoint out of range
+        pm.Pattern.parse("v 0 1\nv 1 1\nv 2 1\ne 0 5 1\n")
+    with pytest.raises(pm.PumatchError):  # vertex 2 has no earlier neighbour
+        pm.Pattern.parse(g_text(4, edges_to_adj(4, [(0, 1), (2, 3), (1, 3)])))
+    with pytest.raises(pm.PumatchError):  # label outside 0..31
+        pm.Pattern.parse("v 0 40\nv 1 1\nv 2 1\ne 0 1 1\ne 1 2 1\n")
+
+
+def test_folding_of_pendant_trees():
+    # labeled K4 + path 3-4-5-6: the path folds onto vertex 3, the core is K4
+    p = pm.Pattern.parse(g_text(7, edges_to_adj(7, C1_EDGES)), True)
+    assert p.info().nslots >= 1
+    # a labeled path folds down to the job edge: still plans
+    pm.Pattern.parse(g_text(4, edges_to_adj(4, [(0, 1), (1, 2), (2, 3)])), True)
+
+
+def test_cli_usage(root):
+    cli = pm.CLI_PATH
+    assert os.path.exists(cli)
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
