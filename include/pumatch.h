@@ -47,8 +47,9 @@ typedef struct {
   int32_t waves;        /* persistent wavefronts (0 = auto: 16 per CU) */
   int32_t chunk;        /* jobs (CSR edges) claimed per atomic (0 = auto) */
   int32_t repeats;      /* timed launches; kernel_ms is their mean (0 = 1) */
-  int32_t split;        /* heavy-root split: vertex-2 candidates per range
-                           task (0 = auto 64, < 0 = no splitting) */
+  int32_t split;        /* heavy-root split: a job's vertex-2 candidates are
+                           cut into range tasks of about `split` list
+                           entries of work (0 = auto 8192, < 0 = off) */
   int32_t hub_factor;   /* fused tail: candidates whose list is longer than
                            hub_factor x the shared set go one by one (0 = 8) */
   int32_t reserved;

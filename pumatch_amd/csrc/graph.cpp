@@ -63,6 +63,28 @@ void compute_max_neighbour_degree(const HostGraph& g, std::vector<uint32_t>* out
   }
 }
 
+void build_label_grouped(const HostGraph& g, std::vector<uint32_t>* lcol,
+                         std::vector<uint32_t>* loff, uint32_t* nl_out) {
+  uint32_t maxl = 0;
+  for (uint32_t l : g.labels) maxl = l > maxl ? l : maxl;
+  const uint32_t nl = maxl + 2;
+  *nl_out = nl;
+  lcol->resize(g.m);
+  loff->assign((size_t)g.n * nl, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t v = 0; v < (int64_t)g.n; v++) {
+    uint32_t* o = loff->data() + (size_t)v * nl;
+    const uint64_t b = g.rowptr[v], e = g.rowptr[v + 1];
+    for (uint64_t i = b; i < e; i++) o[g.labels[g.colidx[i]] + 1]++;
+    for (uint32_t l = 1; l < nl; l++) o[l] += o[l - 1];
+    std::vector<uint32_t> cur(o, o + nl);
+    for (uint64_t i = b; i < e; i++) {
+      const uint32_t x = g.colidx[i];
+      (*lcol)[b + cur[g.labels[x]]++] = x;
+    }
+  }
+}
+
 // pairs = (src << 32 | dst) for directed edges, any order, duplicates and
 // self-loops allowed; builds the sorted, deduplicated, loop-free CSR.
 int build_csr_from_pairs(HostGraph* g, uint64_t n, std::vector<uint64_t>& pairs) {

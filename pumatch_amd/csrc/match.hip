@@ -49,7 +49,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr uint32_t kInf = 0xFFFFFFFFu;
-constexpr int kHashBits = 10;
+constexpr int kHashBits = 9;
 constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
@@ -59,7 +59,8 @@ struct KOp {
   int8_t parent;
   int8_t ncons;
   uint32_t labmask;
-  uint8_t cons_b[8];
+  uint8_t cons_b[7];
+  int8_t seg;  // labeled, one label: read only that label's part of the row
 };
 
 struct KPlan {
@@ -75,7 +76,7 @@ struct KPlan {
   int8_t last_kind;   // 0 reuse slot last_src, 1 view N(phi(k-2)), 2 intersect
   int8_t last_src;
   int8_t last_check;  // labeled: label test needed on the last vertex
-  int8_t pad;
+  int8_t pair;        // fused tail over unordered adjacent pairs (see fused_tail)
   uint32_t split;       // phase-1 split threshold on vertex-2 candidates
   uint32_t hub_factor;  // fused: owners with deg > hub_factor*|P| go one by one
   KOp ops[kMaxSlots];
@@ -88,11 +89,16 @@ struct KGraph {
   const uint32_t* colidx;
   const uint8_t* lab;
   const uint32_t* maxnbr;
+  const uint32_t* lcol;  // rows grouped by neighbour label
+  const uint32_t* loff;
+  uint32_t nl;
+  const uint32_t* src64;  // source vertex of every 64th edge
 };
 
 struct Task {
   unsigned long long edge;
   uint32_t lo, hi;
+  uint32_t r, c;  // the job's endpoints
 };
 
 struct WaveState {
@@ -103,13 +109,14 @@ struct WaveState {
   uint32_t cnext[8];
   uint32_t clen[8];
   uint64_t cmask[8];
-  uint32_t cval[8][kWave];
+  uint32_t cval[6][kWave];  // depths 1..k-3 <= 4
   uint32_t jr[kWave];
   uint32_t jc[kWave];
   // fused step: per-owner data of the current batch
   uint32_t own_off[kWave + 1];
   uint32_t own_ub[kWave];
   unsigned long long own_w[kWave];
+  unsigned long long own_b[kWave];
   uint64_t own_rp[kWave];
   uint32_t hash[kHashCap];
 };
@@ -156,6 +163,27 @@ __device__ __forceinline__ bool member(const uint32_t* a, uint32_t n, uint32_t x
 
 __device__ __forceinline__ uint32_t degree(const KGraph& g, uint32_t v) {
   return (uint32_t)(g.rowptr[v + 1] - g.rowptr[v]);
+}
+
+// N(v), or only its neighbours of label `seg` when seg >= 0 (ascending ids
+// either way); returns the offset into colidx / lcol
+__device__ __forceinline__ uint64_t nbr_list(const KGraph& g, uint32_t v, int seg,
+                                             const uint32_t** base, uint32_t* len) {
+  const uint64_t r0 = g.rowptr[v];
+  if (seg < 0) {
+    *base = g.colidx;
+    *len = (uint32_t)(g.rowptr[v + 1] - r0);
+    return r0;
+  }
+  *base = g.lcol;
+  if ((uint32_t)seg + 1 >= g.nl) {
+    *len = 0;
+    return r0;
+  }
+  const uint32_t* o = g.loff + (size_t)v * g.nl + seg;
+  const uint32_t b = o[0];
+  *len = o[1] - b;
+  return r0 + b;
 }
 
 __device__ __forceinline__ unsigned long long wt_of(const KPlan& P, int u, uint32_t x) {
@@ -261,10 +289,13 @@ __device__ __forceinline__ uint32_t vertex_bound(const KPlan& P, const WaveState
 __device__ void run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, uint32_t* arena,
                         uint32_t cap, uint32_t lane, Counters& ct) {
   const uint32_t v = uni(ws.phi[d]);
-  const uint32_t* nv = uniptr(g.colidx + g.rowptr[v]);
-  const uint32_t dv = uni(degree(g, v));
   for (int o = P.op_begin[d]; o < P.op_end[d]; o++) {
     const KOp& op = P.ops[o];
+    const uint32_t* base;
+    uint32_t dv;
+    const uint64_t off = nbr_list(g, v, op.seg, &base, &dv);
+    const uint32_t* nv = uniptr(base + off);
+    dv = uni(dv);
     const uint32_t ub = P.labeled ? kInf : slot_bound(op, ws);
     const uint32_t nb = (ub == kInf) ? dv : uni(below(nv, dv, ub));
     if (op.kind == 0) {
@@ -276,7 +307,7 @@ __device__ void run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, u
       if (ub != kInf) na = uni(below(a, na, ub));
       uint32_t* out = arena + (size_t)P.region[op.slot] * cap;
       uint32_t cnt;
-      if (P.labeled)
+      if (P.labeled && op.seg < 0)
         cnt = wave_isect<true, true, false>(a, na, nv, nb, op.labmask, g.lab, nullptr, out, lane,
                                             ct, nullptr);
       else
@@ -360,15 +391,18 @@ __device__ unsigned long long last_count(const KGraph& g, const KPlan& P, WaveSt
     na = uni(ws.slen[P.last_src]);
   } else {
     const uint32_t v = uni(ws.phi[dl]);
-    a = uniptr(g.colidx + g.rowptr[v]);
-    na = uni(degree(g, v));
+    const uint32_t* base;
+    const uint64_t off = nbr_list(g, v, P.labeled ? (int)P.lab[u] : -1, &base, &na);
+    a = uniptr(base + off);
+    na = uni(na);
   }
   if (ub != kInf) na = uni(below(a, na, ub));
   if (P.last_kind != 2) {
     if (P.labeled) {
-      if (!P.last_check && !w2) return na;
+      const bool check = P.last_kind == 0 && P.last_check;
+      if (!check && !w2) return na;
       ct.nops++;
-      return wave_label_sum(a, na, P.lab[u], P.last_check, g.lab, w2, lane, ct);
+      return wave_label_sum(a, na, P.lab[u], check, g.lab, w2, lane, ct);
     }
     const uint32_t xm = P.X[u];
     bool in = false;
@@ -381,9 +415,8 @@ __device__ unsigned long long last_count(const KGraph& g, const KPlan& P, WaveSt
   ct.nops++;
   if (P.labeled) {
     unsigned long long s = 0;
-    uint32_t c = wave_isect<false, true, true>(p, np, a, na,
-                                               P.last_check ? (1u << P.lab[u]) : 0xFFFFFFFFu,
-                                               g.lab, w2, nullptr, lane, ct, &s);
+    uint32_t c = wave_isect<false, false, true>(p, np, a, na, 0, g.lab, w2, nullptr, lane, ct,
+                                                &s);
     return w2 ? s : (unsigned long long)c;
   }
   uint32_t cnt = uni(wave_isect<false, false, false>(p, np, a, na, 0, g.lab, nullptr, nullptr,
@@ -511,25 +544,34 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
     if (lane <= (uint32_t)d && ((dropm >> lane) & 1u)) in = member(Pset, nP, ws.phi[lane]);
     inP = (uint32_t)__ballot(in);
   }
-  const bool check2 = lab && P.last_check;
+  const int seg2 = lab ? (int)P.lab[u2] : -1;  // labeled: u2's label part only
+  const bool pair = P.pair;
+  const uint32_t* lbase = lab ? g.lcol : g.colidx;
   const uint64_t hub_lim = isect ? (uint64_t)P.hub_factor * (uint64_t)max(nP, 1u) : ~0ull;
   for (uint32_t base = cbeg; base < cend; base += kWave) {
     const uint32_t i = base + lane;
     uint32_t x = 0, dx = 0, ub = ub_fixed;
-    unsigned long long wx = 0;
+    unsigned long long wx = 0, wb = 0;
     uint64_t rp = 0;
     bool ok = false, hub = false;
     if (i < cend) {
       x = cand[i];
       ok = admit(g, P, ws, u1, P.cand_check[d], x);
       if (ok) {
-        rp = g.rowptr[x];
-        dx = (uint32_t)(g.rowptr[x + 1] - rp);
+        const uint32_t* bb;
+        rp = nbr_list(g, x, seg2, &bb, &dx);
         wx = wd * (w1 ? w1[x] : 1ull);
+        if (pair) {
+          // only partners y > x: the pair {x,y} is counted once, in both roles
+          const uint32_t s0 = below(lbase + rp, dx, x + 1);
+          rp += s0;
+          dx -= s0;
+          wb = wd * (w2 ? w2[x] : 1ull);
+        }
         if (sym_x) ub = min(ub, x);
         if (!lab) {
           uint32_t b = inP;
-          const uint32_t* nx = g.colidx + rp;
+          const uint32_t* nx = lbase + rp;
           while (b) {
             int j = __builtin_ctz(b);
             b &= b - 1;
@@ -551,6 +593,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
     if (lane == kWave - 1) ws.own_off[kWave] = incl;
     ws.own_ub[lane] = ub;
     ws.own_w[lane] = (ok && !hub) ? wx : 0ull;
+    ws.own_b[lane] = (ok && !hub) ? wb : 0ull;
     ws.own_rp[lane] = rp;
     __syncthreads();
     const uint32_t total = uni(ws.own_off[kWave]);
@@ -562,7 +605,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
         const uint32_t mid = (lo + hi) >> 1;
         if (ws.own_off[mid] <= t) lo = mid; else hi = mid;
       }
-      const uint32_t y = g.colidx[ws.own_rp[lo] + (t - ws.own_off[lo])];
+      const uint32_t y = lbase[ws.own_rp[lo] + (t - ws.own_off[lo])];
       if (y >= ws.own_ub[lo]) continue;
       bool hit = true;
       if (isect) {
@@ -575,8 +618,10 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
           hit = member(Pset, nP, y);
         }
       }
-      if (hit && check2) hit = g.lab[y] == P.lab[u2];
-      if (hit) acc += ws.own_w[lo] * (w2 ? w2[y] : 1ull);
+      if (hit) {
+        acc += ws.own_w[lo] * (w2 ? w2[y] : 1ull);
+        if (pair) acc += ws.own_b[lo] * (w1 ? w1[y] : 1ull);
+      }
     }
     __syncthreads();
     // owners whose list is far longer than P: one by one, iterating P and
@@ -588,20 +633,30 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
       const uint32_t xo = uni((uint32_t)__shfl((int)x, o));
       const uint32_t ubo = uni((uint32_t)__shfl((int)ub, o));
       const unsigned long long wo = uni64(__shfl(wx, o));
-      const uint32_t* nx = uniptr(g.colidx + g.rowptr[xo]);
-      uint32_t nxn = uni(degree(g, xo));
+      const unsigned long long wbo = uni64(__shfl(wb, o));
+      const uint32_t* nx = uniptr(lbase + uni64(__shfl(rp, o)));
+      uint32_t nxn = uni((uint32_t)__shfl((int)dx, o));
       if (ubo != kInf) nxn = uni(below(nx, nxn, ubo));
-      const uint32_t nPo = ubo != kInf ? uni(below(Pset, nP, ubo)) : nP;
-      unsigned long long s = 0;
+      const uint32_t* Pp = Pset;
+      uint32_t nPo = ubo != kInf ? uni(below(Pset, nP, ubo)) : nP;
+      if (pair) {  // partners y > x only (nx already starts past x)
+        const uint32_t s0 = uni(below(Pset, nPo, xo + 1));
+        Pp = Pset + s0;
+        nPo -= s0;
+      }
+      unsigned long long s = 0, s1 = 0;
       uint32_t c;
       if (lab)
-        c = wave_isect<false, true, true>(Pset, nPo, nx, nxn,
-                                          P.last_check ? (1u << P.lab[u2]) : 0xFFFFFFFFu, g.lab,
-                                          w2, nullptr, lane, ct, &s);
+        c = wave_isect<false, false, true>(Pp, nPo, nx, nxn, 0, g.lab, w2, nullptr, lane, ct, &s);
       else
-        c = wave_isect<false, false, false>(Pset, nPo, nx, nxn, 0, g.lab, nullptr, nullptr, lane,
+        c = wave_isect<false, false, false>(Pp, nPo, nx, nxn, 0, g.lab, nullptr, nullptr, lane,
                                             ct, nullptr);
-      if (lane == 0) acc += wo * (lab && w2 ? s : (unsigned long long)c);
+      if (pair) {
+        s1 = c;
+        if (w1) wave_isect<false, false, true>(Pp, nPo, nx, nxn, 0, g.lab, w1, nullptr, lane, ct,
+                                               &s1);
+      }
+      if (lane == 0) acc += wo * (lab && w2 ? s : (unsigned long long)c) + (pair ? wbo * s1 : 0ull);
     }
   }
   ct.nops++;
@@ -629,24 +684,80 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
   const uint32_t n2 = uni(ws.clen[1]);
   const uint32_t b = min(lo, n2);
   uint32_t en = min(hi, n2);
-  const uint32_t G = P.split;
-  if (may_split && G > 0 && en > b + G) {
-    // blocks t = 0..want-1 cover [b+G+t*G, ...); the last `fit` go to tasks
-    const uint32_t keep_end = b + G;
-    const uint64_t want = (en - keep_end + G - 1) / G;
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(ntasks, (unsigned long long)want);
-    base = uni64(__shfl(base, 0));
-    const uint64_t fit = base >= task_cap ? 0 : min(want, task_cap - base);
-    const uint64_t first = want - fit;
-    for (uint64_t t = first + lane; t < want; t += kWave) {
-      Task tk;
-      tk.edge = e;
-      tk.lo = (uint32_t)(keep_end + t * G);
-      tk.hi = (uint32_t)min((uint64_t)keep_end + (t + 1) * G, (uint64_t)en);
-      tasks[base + (t - first)] = tk;
+  const uint32_t W = P.split;
+  if (may_split && W > 0 && en > b + 1) {
+    // Work-aware split of the vertex-2 candidates.  Candidate x costs
+    // 1 + |list the next level reads from x|; a new range starts at every
+    // candidate whose exclusive running cost crosses a multiple of W.  The
+    // wave keeps the first range and writes the others as tasks (hi = 0:
+    // up to the next task's lo; hi = ~0: to the end of the list).
+    const uint32_t* cand = uniptr(ws.sptr[P.cand_slot[1]]);
+    const int cseg = (P.labeled && P.k == 4) ? (int)P.lab[3] : -1;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint64_t nstart = 0;
+    uint32_t keep_end = en;
+    unsigned long long tb = 0;
+    bool write = false;
+    for (int pass = 0; pass < 2; pass++) {
+      uint64_t run = 0, prev_c = 0, rank0 = 0;
+      for (uint32_t base = b; base < en; base += kWave) {
+        const uint32_t i = base + lane;
+        uint64_t cost = 0;
+        if (i < en) {
+          const uint32_t* bb;
+          uint32_t len;
+          nbr_list(g, cand[i], cseg, &bb, &len);
+          cost = 1ull + len;
+        }
+        uint64_t incl = cost;
+        for (int o = 1; o < kWave; o <<= 1) {
+          const uint64_t t = __shfl_up(incl, o);
+          if ((int)lane >= o) incl += t;
+        }
+        const uint64_t before = run + incl - cost;
+        const uint64_t up = __shfl_up(cost, 1);
+        const uint64_t cprev = lane > 0 ? up : prev_c;
+        const bool starts = i < en && i > b && (before / W) != ((before - cprev) / W);
+        const uint64_t sm = __ballot(starts);
+        if (pass == 1 && write && starts) {
+          const uint64_t rank = rank0 + __popcll(sm & lt);
+          Task tk;
+          tk.edge = e;
+          tk.lo = i;
+          tk.hi = rank + 1 == nstart ? 0xFFFFFFFFu : 0u;
+          tk.r = r;
+          tk.c = c;
+          tasks[tb + rank] = tk;
+        }
+        if (pass == 1 && sm && rank0 == 0 && keep_end == en)
+          keep_end = uni((uint32_t)__shfl((int)i, __builtin_ctzll(sm)));
+        rank0 += __popcll(sm);
+        run += uni64(wave_sum(cost));
+        const uint32_t ll = min(en - base, (uint32_t)kWave) - 1u;
+        prev_c = uni64(__shfl(cost, ll));
+      }
+      if (pass == 0) {
+        nstart = rank0;
+        if (nstart == 0) break;
+        if (lane == 0) tb = atomicAdd(ntasks, (unsigned long long)nstart);
+        tb = uni64(__shfl(tb, 0));
+        write = tb + nstart <= task_cap;
+        if (!write) {
+          // buffer full: blank the reserved slots inside it, keep all work
+          for (uint64_t t = tb + lane; t < min(tb + nstart, task_cap); t += kWave) {
+            Task tk;
+            tk.edge = e;
+            tk.lo = 1;
+            tk.hi = 1;
+            tk.r = r;
+            tk.c = c;
+            tasks[t] = tk;
+          }
+          break;
+        }
+      }
     }
-    en = (uint32_t)min((uint64_t)en, (uint64_t)keep_end + first * G);
+    if (write) en = keep_end;
   }
   ws.cnext[1] = b;
   ws.clen[1] = en;
@@ -689,13 +800,9 @@ __device__ __forceinline__ uint32_t source_of(const KGraph& g, uint64_t e, uint3
   return lo;
 }
 
+// source of edge e: start at the indexed source of edge 64*(e/64), gallop
 __device__ __forceinline__ uint32_t source_full(const KGraph& g, uint64_t e) {
-  uint32_t lo = 0, hi = (uint32_t)g.n;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (g.rowptr[mid] <= e) lo = mid; else hi = mid;
-  }
-  return lo;
+  return source_of(g, e, g.src64[e >> 6]);
 }
 
 struct Launch {
@@ -713,7 +820,7 @@ struct Launch {
   uint64_t task_cap;
 };
 
-__global__ __launch_bounds__(64) void k_match_edges(Launch L) {
+__global__ __launch_bounds__(64, 6) void k_match_edges(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
   const KPlan& P = *L.plan;
@@ -771,7 +878,7 @@ __global__ __launch_bounds__(64) void k_match_edges(Launch L) {
   }
 }
 
-__global__ __launch_bounds__(64) void k_match_tasks(Launch L) {
+__global__ __launch_bounds__(64, 6) void k_match_tasks(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
   const KPlan& P = *L.plan;
@@ -786,9 +893,11 @@ __global__ __launch_bounds__(64) void k_match_tasks(Launch L) {
     t = uni64(__shfl(t, 0));
     if (t >= ntask) break;
     const uint64_t e = uni64(L.tasks[t].edge);
-    const uint32_t tlo = uni(L.tasks[t].lo), thi = uni(L.tasks[t].hi);
-    const uint32_t r = uni(source_full(g, e));
-    const uint32_t c = uni(g.colidx[e]);
+    const uint32_t tlo = uni(L.tasks[t].lo);
+    uint32_t thi = uni(L.tasks[t].hi);
+    if (thi == 0) thi = uni(L.tasks[t + 1].lo);  // same job: slots are contiguous
+    const uint32_t r = uni(L.tasks[t].r);
+    const uint32_t c = uni(L.tasks[t].c);
     const unsigned long long cnt =
         run_job(g, P, ws, arena, L.cap, lane, r, c, e, tlo, thi, false, nullptr, nullptr, 0, ct);
     total += cnt;
@@ -801,10 +910,11 @@ __global__ __launch_bounds__(64) void k_match_tasks(Launch L) {
   }
 }
 
-// F(v) = sum over x in N(v) with lab[x] == want of prod_c child_c[x]: one
-// wave per group of 64 consecutive rows, walking the group's edges 64 at a
-// time (coalesced colidx reads); each lane finds its row in the LDS copy of
-// the group's row offsets and adds into an LDS accumulator.
+// F(v) = sum over x in N(v) with lab[x] == want of prod_c child_c[x].
+// Edge-balanced: each wave walks 512 consecutive CSR edges per claim, 64 at
+// a time; lanes of the same row are summed with a segmented wave scan and
+// the last lane of each row segment adds into out[row] (zeroed before), so
+// a hub row is spread over many waves instead of serialising one.
 struct FoldArgs {
   KGraph g;
   uint32_t want;
@@ -814,48 +924,46 @@ struct FoldArgs {
   unsigned long long* bytes;  // algorithmic bytes accumulator
 };
 
+constexpr uint32_t kFoldChunk = 512;
+
 __global__ __launch_bounds__(64) void k_fold(FoldArgs A) {
-  __shared__ uint64_t rp[kWave + 1];
-  __shared__ unsigned long long acc[kWave];
   const KGraph& g = A.g;
   const uint32_t lane = threadIdx.x;
-  const uint64_t ngroups = (g.n + kWave - 1) / kWave;
-  unsigned long long hits = 0, edges = 0, rows = 0;
-  for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    const uint64_t v0 = grp * kWave;
-    const uint32_t nv = (uint32_t)min((uint64_t)kWave, g.n - v0);
-    rp[lane] = g.rowptr[v0 + min(lane, nv)];
-    if (lane == 0) rp[kWave] = g.rowptr[v0 + nv];
-    acc[lane] = 0;
-    __syncthreads();
-    const uint64_t e0 = rp[0], e1 = rp[kWave];
-    for (uint64_t eb = e0; eb < e1; eb += kWave) {
+  unsigned long long hits = 0, edges = 0;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * kFoldChunk; c0 < g.m;
+       c0 += (uint64_t)gridDim.x * kFoldChunk) {
+    const uint64_t c1 = min(c0 + kFoldChunk, g.m);
+    uint32_t r = uni(source_full(g, c0));
+    for (uint64_t eb = c0; eb < c1; eb += kWave) {
       const uint64_t e = eb + lane;
-      if (e < e1) {
-        uint32_t lo = 0, hi = kWave;
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (rp[mid] <= e) lo = mid; else hi = mid;
-        }
+      uint32_t row = 0xFFFFFFFFu;
+      unsigned long long val = 0;
+      if (e < c1) {
+        row = source_of(g, e, r);
         const uint32_t x = g.colidx[e];
         if (g.lab[x] == A.want) {
-          unsigned long long p = 1;
-          for (int c = 0; c < A.nchild; c++) p *= A.child[c][x];
-          atomicAdd(&acc[lo], p);
+          val = 1;
+          for (int c = 0; c < A.nchild; c++) val *= A.child[c][x];
           hits++;
         }
       }
+      // rows are non-decreasing across lanes: segmented inclusive scan
+      for (int o = 1; o < kWave; o <<= 1) {
+        const unsigned long long vv = __shfl_up(val, o);
+        const uint32_t rr = (uint32_t)__shfl_up((int)row, o);
+        if ((int)lane >= o && rr == row) val += vv;
+      }
+      const uint32_t next = (uint32_t)__shfl_down((int)row, 1);
+      if (row != 0xFFFFFFFFu && (lane == kWave - 1 || next != row) && val)
+        atomicAdd(&A.out[row], val);
+      const uint32_t lastl = (uint32_t)min((uint64_t)kWave, c1 - eb) - 1u;
+      r = uni((uint32_t)__shfl((int)row, (int)lastl));
     }
-    __syncthreads();
-    if (lane < nv) A.out[v0 + lane] = acc[lane];
-    edges += e1 - e0;
-    rows += nv;
-    __syncthreads();
+    edges += c1 - c0;
   }
-  // colidx 4 B + label 1 B per edge, 8 B per child value gathered, row
-  // offsets 8 B and the result 8 B per row
+  // colidx 4 B + label 1 B per edge, 8 B per child value gathered
   hits = wave_sum(hits);
-  if (lane == 0) atomicAdd(A.bytes, 5ull * edges + 8ull * A.nchild * hits + 16ull * rows);
+  if (lane == 0) atomicAdd(A.bytes, 5ull * edges + 8ull * A.nchild * hits);
 }
 
 // out[v] = prod_c child_c[v]
@@ -898,6 +1006,7 @@ struct Workspace {
   size_t table_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, eve = nullptr;
   int cus = 0;
+  int resident = 0;
 };
 std::mutex g_ws_mu;
 Workspace g_ws;
@@ -959,16 +1068,22 @@ void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
         ko.ncons = (int8_t)op.ncons;
         ko.labmask = op.labmask;
         for (int c = 0; c < op.ncons; c++) ko.cons_b[c] = op.cons_b[c];
+        ko.seg = (p.labeled && __builtin_popcount(op.labmask) == 1)
+                     ? (int8_t)__builtin_ctz(op.labmask)
+                     : (int8_t)-1;
         if (op.kind == 1) kp->region[op.slot] = (int8_t)nreg++;
       }
     }
     kp->op_end[d] = (int8_t)nops;
   }
   kp->nops = nops;
+  // a slot holds only label-`label` vertices when its label filter is that
+  // one label (INTERSECT filters, VIEW reads that label's part of the row)
   auto exact_slot = [&](int s, int label, int upto) {
+    if (!p.labeled) return false;
     for (int dd = 0; dd <= upto; dd++)
       for (const SlotOp& op : p.ops[dd])
-        if (op.slot == s && op.kind == 1 && op.labmask == (1u << label)) return true;
+        if (op.slot == s && op.labmask == (1u << label)) return true;
     return false;
   };
   for (int d = 1; d <= kc - 3; d++) {
@@ -986,13 +1101,42 @@ void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
       const SlotOp& op = p.ops[dl][0];
       kp->last_kind = op.kind == 0 ? 1 : 2;
       kp->last_src = op.parent;
-      kp->last_check = (op.kind == 1 && exact_slot(op.parent, p.clabel[kc - 1], dl - 1)) ? 0 : 1;
+      kp->last_check = 0;  // read from the last vertex's label part of the row
     }
+    // Labeled tail whose two vertices are interchangeable: u1's candidates
+    // are the set P that u2 intersects with N(x), both label-exact with one
+    // label.  Then the ordered pairs (x,y) are the adjacent pairs inside P,
+    // and each unordered pair can be visited once (y > x) and credited in
+    // both roles.
+    if (p.labeled && kc >= 4 && kp->last_kind == 2 && kp->last_src == kp->cand_slot[kc - 3] &&
+        p.clabel[kc - 2] == p.clabel[kc - 1] && kp->cand_check[kc - 3] == 0)
+      kp->pair = 1;
   }
   *nregions = nreg;
 }
 
 }  // namespace
+
+// (Re)build the label-grouped copy of the rows when the graph has labels.
+int upload_label_grouped(pm_graph* g) {
+  DeviceGraph& d = g->d;
+  if (d.lcol) HIP_OK(hipFree(d.lcol));
+  if (d.loff) HIP_OK(hipFree(d.loff));
+  d.lcol = nullptr;
+  d.loff = nullptr;
+  d.nl = 0;
+  const HostGraph& h = g->h;
+  if (h.labels.empty() || h.n == 0) return PM_OK;
+  std::vector<uint32_t> lcol, loff;
+  uint32_t nl = 0;
+  build_label_grouped(h, &lcol, &loff, &nl);
+  HIP_OK(hipMalloc(&d.lcol, sizeof(uint32_t) * std::max<uint64_t>(h.m, 1)));
+  HIP_OK(hipMalloc(&d.loff, sizeof(uint32_t) * loff.size()));
+  if (h.m) HIP_OK(hipMemcpy(d.lcol, lcol.data(), sizeof(uint32_t) * h.m, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d.loff, loff.data(), sizeof(uint32_t) * loff.size(), hipMemcpyHostToDevice));
+  d.nl = nl;
+  return PM_OK;
+}
 
 int device_upload(pm_graph* g) {
   if (g->on_device) return PM_OK;
@@ -1017,8 +1161,22 @@ int device_upload(pm_graph* g) {
     HIP_OK(hipMemcpy(d.label8, lab8.data(), h.n, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d.maxnbr, maxnbr.data(), sizeof(uint32_t) * h.n, hipMemcpyHostToDevice));
   }
+  {
+    // source of every 64th edge: edge -> source lookups start here
+    const uint64_t ns = (h.m + 63) / 64 + 1;
+    std::vector<uint32_t> src64(ns, 0);
+    uint64_t v = 0;
+    for (uint64_t i = 0; i * 64 < h.m; i++) {
+      while (h.rowptr[v + 1] <= i * 64) v++;
+      src64[i] = (uint32_t)v;
+    }
+    HIP_OK(hipMalloc(&d.src64, sizeof(uint32_t) * ns));
+    HIP_OK(hipMemcpy(d.src64, src64.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+  }
   d.bytes = sizeof(uint64_t) * (h.n + 1) + sizeof(uint32_t) * h.m + h.n + 4 * h.n;
   g->on_device = true;
+  rc = upload_label_grouped(g);
+  if (rc) return rc;
   g->device = dev;
   return PM_OK;
 }
@@ -1029,7 +1187,7 @@ int device_update_labels(pm_graph* g) {
   std::vector<uint8_t> lab8(h.n, 0);
   for (uint64_t v = 0; v < h.n && !h.labels.empty(); v++) lab8[v] = (uint8_t)h.labels[v];
   if (h.n) HIP_OK(hipMemcpy(g->d.label8, lab8.data(), h.n, hipMemcpyHostToDevice));
-  return PM_OK;
+  return upload_label_grouped(g);
 }
 
 void device_release(pm_graph* g) {
@@ -1038,6 +1196,9 @@ void device_release(pm_graph* g) {
   (void)hipFree(g->d.colidx);
   (void)hipFree(g->d.label8);
   (void)hipFree(g->d.maxnbr);
+  if (g->d.src64) (void)hipFree(g->d.src64);
+  if (g->d.lcol) (void)hipFree(g->d.lcol);
+  if (g->d.loff) (void)hipFree(g->d.loff);
   g->d = DeviceGraph();
   g->on_device = false;
 }
@@ -1057,7 +1218,13 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   int nreg = 0;
   build_kplan(plan, &kp, &nreg);
   hipStream_t st = opts->stream ? (hipStream_t)opts->stream : g_ws.stream;
-  const int waves = opts->waves > 0 ? opts->waves : g_ws.cus * 16;
+  // default: every wave the chip can hold at once (persistent grid)
+  if (g_ws.resident == 0) {
+    int per_cu = 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_match_edges, kWave, 0));
+    g_ws.resident = std::max(1, per_cu) * g_ws.cus;
+  }
+  const int waves = opts->waves > 0 ? opts->waves : g_ws.resident;
   const uint64_t lo = opts->edge_lo, hi = opts->edge_hi ? opts->edge_hi : h.m;
   if (hi > h.m || lo > hi) {
     set_error("edge range outside the graph");
@@ -1067,7 +1234,7 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
       opts->chunk > 0 ? (uint32_t)opts->chunk
                       : (uint32_t)std::max<uint64_t>(
                             1, std::min<uint64_t>(64, (hi - lo) / ((uint64_t)waves * 8)));
-  kp.split = opts->split > 0 ? (uint32_t)opts->split : 64u;
+  kp.split = opts->split > 0 ? (uint32_t)opts->split : 8192u;
   kp.hub_factor = opts->hub_factor > 0 ? (uint32_t)opts->hub_factor : 8u;
 
   // fold / weight tables: one u64[n] per folded vertex and per weighted
@@ -1094,7 +1261,8 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   unsigned long long* d_per_edge = nullptr;
   if (opts->per_edge && hi > lo)
     HIP_OK(hipMallocAsync((void**)&d_per_edge, sizeof(unsigned long long) * (hi - lo), st));
-  const KGraph kg{h.n, h.m, g->d.rowptr, g->d.colidx, g->d.label8, g->d.maxnbr};
+  const KGraph kg{h.n,         h.m,       g->d.rowptr, g->d.colidx, g->d.label8, g->d.maxnbr,
+                  g->d.lcol,   g->d.loff, g->d.nl,     g->d.src64};
   Launch L;
   L.g = kg;
   L.plan = g_ws.plan;
@@ -1110,8 +1278,8 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   L.task_cap = g_ws.task_cap;
   const int reps = opts->repeats > 0 ? opts->repeats : 1;
   double sum_ms = 0, min_ms = 1e30, phase[3] = {0, 0, 0};
-  const int fold_blocks =
-      (int)std::max<uint64_t>(1, std::min<uint64_t>((h.n + kWave - 1) / kWave, (uint64_t)g_ws.cus * 32));
+  const int fold_blocks = (int)std::max<uint64_t>(
+      1, std::min<uint64_t>((h.m + kFoldChunk - 1) / kFoldChunk, (uint64_t)g_ws.cus * 16));
   for (int r = 0; r < reps; r++) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor, 8, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.out, 8, 0ull);
@@ -1130,7 +1298,10 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
         if (plan.fold_children[u] >> c & 1u) A.child[A.nchild++] = table(table_of[c]);
       A.out = table(table_of[u]);
       A.bytes = g_ws.out + 6;
-      if (h.n) hipLaunchKernelGGL(k_fold, dim3(fold_blocks), dim3(kWave), 0, st, A);
+      if (h.n) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, st, A.out, h.n, 0ull);
+        if (h.m) hipLaunchKernelGGL(k_fold, dim3(fold_blocks), dim3(kWave), 0, st, A);
+      }
     }
     for (int i = 0; i < plan.kc; i++) {
       if (core_tab[i] < 0) continue;

@@ -27,11 +27,18 @@ struct HostGraph {
 // Device-resident copy (HBM).  label8 holds the raw label (0..31) as a byte,
 // maxnbr[v] = max degree over N(v) (evaluates the job filter's
 // "exists w in N(c) with deg(w) >= d_P(2)" in O(1)).
+// lcol/loff: the same rows regrouped by neighbour label (ids ascending
+// within a label); the label-l part of row v is
+// lcol[rowptr[v] + loff[v*nl + l] .. rowptr[v] + loff[v*nl + l + 1]).
 struct DeviceGraph {
   uint64_t* rowptr = nullptr;
   uint32_t* colidx = nullptr;
   uint8_t* label8 = nullptr;
   uint32_t* maxnbr = nullptr;
+  uint32_t* lcol = nullptr;
+  uint32_t* loff = nullptr;
+  uint32_t nl = 0;  // labels + 1 (0: no label-grouped copy)
+  uint32_t* src64 = nullptr;  // source vertex of edge 64*i
   size_t bytes = 0;
 };
 
@@ -119,6 +126,10 @@ void assign_uniform_labels(HostGraph* g, uint32_t nlabels, uint64_t seed);
 int build_csr_from_pairs(HostGraph* g, uint64_t n, std::vector<uint64_t>& pairs);
 void finalize_graph(HostGraph* g);
 void compute_max_neighbour_degree(const HostGraph& g, std::vector<uint32_t>* out);
+// rows regrouped by neighbour label (stable, so ids stay ascending within a
+// label); nl = max label + 2
+void build_label_grouped(const HostGraph& g, std::vector<uint32_t>* lcol,
+                         std::vector<uint32_t>* loff, uint32_t* nl);
 
 // device (match.hip)
 int device_upload(pm_graph* g);
