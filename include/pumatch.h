@@ -36,6 +36,11 @@ enum {
   PM_ERR_NOMEM = -7
 };
 
+/* out-of-core cache replacement policies */
+#define PM_POLICY_LRU 0     /* least recently used slot (lruAccess_bucket, src/buffer.cu:581-628) */
+#define PM_POLICY_FIFO 1    /* oldest load first */
+#define PM_POLICY_LEARNED 2 /* lowest predicted reuse first (pm_model scores), then LRU */
+
 /* graph flags */
 #define PM_GRAPH_LABELS 0x1    /* also read <prefix>.label.bin */
 #define PM_GRAPH_HOST_ONLY 0x2 /* keep the CSR on the host (no HBM copy) */
@@ -52,11 +57,17 @@ typedef struct {
                            entries of work (0 = auto 8192, < 0 = off) */
   int32_t hub_factor;   /* fused tail: candidates whose list is longer than
                            hub_factor x the shared set go one by one (0 = 8) */
-  int32_t reserved;
+  int32_t policy;       /* out-of-core replacement policy (PM_POLICY_*) */
   uint64_t edge_lo;     /* restrict jobs to CSR edges [edge_lo, edge_hi); hi=0: all */
   uint64_t edge_hi;
   uint64_t* per_edge;   /* optional host u64[m]: count per job edge (before x|Aut|) */
   void* stream;         /* hipStream_t to launch on (NULL: the library's own) */
+  uint64_t cache_bytes; /* out-of-core: rows stay in pinned host memory and are
+                           paged into an HBM cache of this size (0 = rows
+                           resident in HBM).  Replaces the package buffer,
+                           src/buffer.cu:264-415,800-929 */
+  uint32_t package_bytes; /* cache slot / package size (0 = 2 MiB) */
+  int32_t prefetch;     /* out-of-core: load predicted packages before each round */
 } pm_opts;
 
 typedef struct {
@@ -66,7 +77,7 @@ typedef struct {
   uint64_t jobs;        /* jobs that passed the filter (src/job_queue.h:44-61) */
   uint64_t alg_bytes;   /* algorithmic bytes of one launch (see DESIGN) */
   uint64_t set_ops;     /* set operations executed by one launch */
-  uint64_t link_bytes;  /* host->HBM bytes paged during the launch */
+  uint64_t link_bytes;  /* host->HBM bytes paged during the run (out-of-core) */
   uint64_t tasks;       /* range tasks created by heavy-root splitting */
   int32_t waves;        /* wavefronts actually launched */
   int32_t chunk;
@@ -74,6 +85,11 @@ typedef struct {
      stream): 0 pendant-tree folding, 1 edge phase, 2 range-task phase */
   double phase_ms[3];
   uint64_t phase_alg_bytes[3];
+  uint64_t misses;      /* out-of-core: package misses over all rounds */
+  uint64_t deferred;    /* out-of-core: branches deferred and resumed */
+  int32_t rounds;       /* out-of-core: matching rounds (1 when resident) */
+  int32_t packages;     /* out-of-core: packages in the graph */
+  double link_ms;       /* out-of-core: host->HBM copy time */
 } pm_result;
 
 typedef struct {

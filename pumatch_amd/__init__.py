@@ -38,11 +38,14 @@ class Opts(C.Structure):
         ("repeats", C.c_int32),
         ("split", C.c_int32),
         ("hub_factor", C.c_int32),
-        ("reserved", C.c_int32),
+        ("policy", C.c_int32),
         ("edge_lo", C.c_uint64),
         ("edge_hi", C.c_uint64),
         ("per_edge", C.POINTER(C.c_uint64)),
         ("stream", C.c_void_p),
+        ("cache_bytes", C.c_uint64),
+        ("package_bytes", C.c_uint32),
+        ("prefetch", C.c_int32),
     ]
 
 
@@ -60,6 +63,11 @@ class Result(C.Structure):
         ("chunk", C.c_int32),
         ("phase_ms", C.c_double * 3),
         ("phase_alg_bytes", C.c_uint64 * 3),
+        ("misses", C.c_uint64),
+        ("deferred", C.c_uint64),
+        ("rounds", C.c_int32),
+        ("packages", C.c_int32),
+        ("link_ms", C.c_double),
     ]
 
 
@@ -287,17 +295,28 @@ class MatchResult:
     chunk: int
     phase_ms: tuple = ()
     phase_alg_bytes: tuple = ()
+    misses: int = 0
+    deferred: int = 0
+    rounds: int = 1
+    packages: int = 0
+    link_ms: float = 0.0
     per_edge: np.ndarray | None = None
+
+
+POLICY = {"lru": 0, "fifo": 1, "learned": 2}
 
 
 def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int = 1,
           split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
-          per_edge: bool = False, stream: int | None = None) -> MatchResult:
+          per_edge: bool = False, stream: int | None = None, cache_bytes: int = 0,
+          package_bytes: int = 0, policy: str = "lru", prefetch: bool = False) -> MatchResult:
     """Count matches of p in g on the GPU (pm_match)."""
     o = Opts()
     lib().pm_opts_default(C.byref(o))
     o.waves, o.chunk, o.repeats = waves, chunk, repeats
     o.split, o.hub_factor = split, hub_factor
+    o.cache_bytes, o.package_bytes = cache_bytes, package_bytes
+    o.policy, o.prefetch = POLICY[policy], 1 if prefetch else 0
     o.edge_lo, o.edge_hi = edge_lo, edge_hi
     pe = None
     if per_edge:
@@ -309,7 +328,9 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
     _check(lib().pm_match(g._h, p._h, C.byref(o), C.byref(r)), "match")
     return MatchResult(r.kernel_ms, r.min_ms, int(r.count), int(r.jobs), int(r.alg_bytes),
                        int(r.set_ops), int(r.link_bytes), int(r.tasks), int(r.waves), int(r.chunk),
-                       tuple(r.phase_ms), tuple(int(x) for x in r.phase_alg_bytes), pe)
+                       tuple(r.phase_ms), tuple(int(x) for x in r.phase_alg_bytes),
+                       int(r.misses), int(r.deferred), int(r.rounds), int(r.packages),
+                       float(r.link_ms), pe)
 
 
 def pattern_text(k: int, edges, labels=None) -> str:

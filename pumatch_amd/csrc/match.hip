@@ -1,4 +1,4 @@
-// Subgraph-matching kernels for gfx950 (CDNA4, wave64).
+// Subgraph-matching kernels for gfx950 (CDNA4, wave64) and their host driver.
 //
 // One persistent wavefront runs one depth-first search at a time (the
 // reference: one warp per DFS, _parallel_match / match / extend,
@@ -8,13 +8,13 @@
 // src/gpu_match.cu:321-384 and the host job scan src/job_queue.h:39-89):
 //   phase 1  persistent waves claim chunks of CSR edges with one global
 //            atomic; the job filter of src/job_queue.h:44-61 is evaluated
-//            lane-parallel; a job whose vertex-2 candidate list is longer
-//            than `split` keeps the first block and appends the rest as
-//            range tasks (edge, [lo,hi)) to a task buffer;
+//            lane-parallel; a job whose vertex-2 candidates carry more than
+//            `split` list entries of estimated work keeps its first range
+//            and writes the rest as range tasks;
 //   phase 2  persistent waves claim range tasks, rebuild the depth-0/1 sets
-//            of the task's edge and finish the range.  This is the heavy-
-//            root splitting the reference approximates with stealing
-//            (trans_layer, src/gpu_match.cu:21-109, disabled by default).
+//            of the task's job and finish the range.  This is the heavy-root
+//            splitting the reference approximates with stealing (trans_layer,
+//            src/gpu_match.cu:21-109, disabled by default).
 //
 // Set operations (src/gpu_match.cu:238-319): sorted-list intersections,
 // each lane taking one element of the shorter list and searching the longer
@@ -24,23 +24,38 @@
 // src/gpu_match.cu:302,316).
 //
 // The last two core vertices are fused (the reference enumerates vertex k-2
-// one candidate at a time and counts vertex k-1, :1018-1041): all candidates
-// of vertex k-2 are taken 64 at a time and the matches of vertex k-1 for
-// each are counted lane-parallel; when that count is an intersection with a
+// one candidate at a time and counts vertex k-1, :1018-1041): candidates of
+// vertex k-2 are taken 64 at a time and the matches of vertex k-1 for each
+// are counted lane-parallel; when that count is an intersection with a
 // shared parent set P, the candidates' neighbour lists are walked
 // edge-parallel and membership in P is an LDS hash probe.
 //
 // Labeled (homomorphism) counting folds pendant trees of the pattern into
 // per-vertex weight tables built by an edge-parallel sparse pass (k_fold);
 // the DFS runs over the remaining core and multiplies weights.
+//
+// Out of core (the package buffer of src/buffer.cu and the missing-branch
+// tree of src/missing_tree.cu): rows stay in pinned host memory, cut into
+// packages that are paged into a fixed HBM cache between matching rounds.
+// A wave that needs a row whose package is not resident records the package
+// (deduplicated bitmap + list), writes a resume descriptor for exactly the
+// point it reached (job, candidate index per depth) and returns with only
+// completed subtrees counted.  The host loads the missed packages
+// (hipMemcpyAsync on a side stream into policy-chosen victims) and the next
+// round resumes every descriptor.  Nothing is dropped, so counts are exact
+// (the reference replays abandoned branches at most REPEATED_CNT times from
+// a 1024-node tree, src/gpu_match.cu:926,969, src/missing_tree.cu:46).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
+#include "paging.hpp"
 #include "pm_internal.hpp"
 
 namespace pm {
@@ -52,6 +67,7 @@ constexpr uint32_t kInf = 0xFFFFFFFFu;
 constexpr int kHashBits = 9;
 constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kMaxDepth = 6;
 
 struct KOp {
   int8_t kind;  // 0 VIEW, 1 INTERSECT
@@ -77,7 +93,9 @@ struct KPlan {
   int8_t last_src;
   int8_t last_check;  // labeled: label test needed on the last vertex
   int8_t pair;        // fused tail over unordered adjacent pairs (see fused_tail)
-  uint32_t split;       // phase-1 split threshold on vertex-2 candidates
+  int8_t segs;        // label-grouped rows available (labeled, resident)
+  int8_t pad[3];
+  uint32_t split;       // phase-1 split threshold (list entries of work)
   uint32_t hub_factor;  // fused: owners with deg > hub_factor*|P| go one by one
   KOp ops[kMaxSlots];
   const unsigned long long* wt[8];  // weight table per core vertex (nullptr = 1)
@@ -86,19 +104,54 @@ struct KPlan {
 struct KGraph {
   uint64_t n, m;
   const uint64_t* rowptr;
-  const uint32_t* colidx;
+  const uint32_t* colidx;  // rows in HBM (nullptr when paged)
+  const uint32_t* jcol;    // rows for the edge scan (HBM, or mapped host memory)
   const uint8_t* lab;
   const uint32_t* maxnbr;
   const uint32_t* lcol;  // rows grouped by neighbour label
   const uint32_t* loff;
   uint32_t nl;
   const uint32_t* src64;  // source vertex of every 64th edge
+  // out of core (pkg_slot == nullptr: rows resident)
+  const int32_t* pkg_slot;
+  const uint32_t* pkg_of;
+  const uint64_t* pkg_base;
+  const uint32_t* cache;
+  uint64_t slot_words;
+  unsigned int* miss_bits;
+  uint32_t* miss_list;
+  unsigned long long* miss_count;
+  uint32_t miss_cap;
+  uint8_t* used;  // per cache slot: read during this round
 };
 
 struct Task {
   unsigned long long edge;
   uint32_t lo, hi;
   uint32_t r, c;  // the job's endpoints
+};
+
+// A deferred piece of search.  depth 0: the whole job (edge, r, c) over its
+// vertex-2 range [lo, hi).  depth D >= 1: the subtrees of candidates
+// [idx[D], end) at depth D below the path given by the candidate indices
+// idx[1..D-1]; phi(0..D) are that path's vertices (their rows are needed to
+// rebuild it) and blk is the package whose absence deferred the piece.
+struct Resume {
+  unsigned long long edge;
+  uint32_t r, c, lo, hi;
+  int32_t depth;
+  uint32_t end;
+  uint32_t idx[kMaxDepth];
+  uint32_t phi[kMaxDepth];
+  uint32_t blk;
+  uint32_t pad;
+};
+
+struct DeferOut {
+  Resume* buf;
+  unsigned long long* count;
+  uint64_t cap;
+  uint64_t soft;  // beyond this many pieces, searches stop at their first miss
 };
 
 struct WaveState {
@@ -108,8 +161,10 @@ struct WaveState {
   unsigned long long wacc[8];
   uint32_t cnext[8];
   uint32_t clen[8];
+  uint32_t cbase[8];
+  uint32_t cidx[8];  // index of the candidate being expanded at each depth
   uint64_t cmask[8];
-  uint32_t cval[6][kWave];  // depths 1..k-3 <= 4
+  uint32_t cval[kMaxDepth][kWave];  // depths 1..k-3 <= 4
   uint32_t jr[kWave];
   uint32_t jc[kWave];
   // fused step: per-owner data of the current batch
@@ -117,8 +172,14 @@ struct WaveState {
   uint32_t own_ub[kWave];
   unsigned long long own_w[kWave];
   unsigned long long own_b[kWave];
-  uint64_t own_rp[kWave];
+  uint64_t own_ptr[kWave];  // row address of each owner (as an integer)
   uint32_t hash[kHashCap];
+  unsigned long long je;  // the job being searched: edge and vertex-2 range
+  uint32_t jlo, jhi;
+  uint32_t blk;     // package of the last row found missing
+  int32_t rdepth;   // resume input: depth, candidate indices, range end
+  uint32_t ridx[kMaxDepth];
+  uint32_t rend;
 };
 
 struct Counters {
@@ -165,25 +226,53 @@ __device__ __forceinline__ uint32_t degree(const KGraph& g, uint32_t v) {
   return (uint32_t)(g.rowptr[v + 1] - g.rowptr[v]);
 }
 
-// N(v), or only its neighbours of label `seg` when seg >= 0 (ascending ids
-// either way); returns the offset into colidx / lcol
-__device__ __forceinline__ uint64_t nbr_list(const KGraph& g, uint32_t v, int seg,
-                                             const uint32_t** base, uint32_t* len) {
-  const uint64_t r0 = g.rowptr[v];
-  if (seg < 0) {
-    *base = g.colidx;
-    *len = (uint32_t)(g.rowptr[v + 1] - r0);
-    return r0;
-  }
-  *base = g.lcol;
-  if ((uint32_t)seg + 1 >= g.nl) {
-    *len = 0;
-    return r0;
-  }
+// Length of N(v) (or its label-`seg` part) without touching the row itself.
+__device__ __forceinline__ uint32_t row_len(const KGraph& g, uint32_t v, int seg) {
+  if (seg < 0) return degree(g, v);
+  if ((uint32_t)seg + 1 >= g.nl) return 0;
   const uint32_t* o = g.loff + (size_t)v * g.nl + seg;
-  const uint32_t b = o[0];
-  *len = o[1] - b;
-  return r0 + b;
+  return o[1] - o[0];
+}
+
+// N(v), or only its neighbours of label `seg` when seg >= 0 (ascending ids
+// either way).  Out of core, a row whose package is not resident is recorded
+// as a miss and false is returned.  Lane-local.
+template <bool PG>
+__device__ __forceinline__ bool rows(const KGraph& g, uint32_t v, int seg, const uint32_t** p,
+                                     uint32_t* len, uint32_t* missing = nullptr) {
+  const uint64_t r0 = g.rowptr[v];
+  if (!PG) {
+    if (seg < 0) {
+      *p = g.colidx + r0;
+      *len = (uint32_t)(g.rowptr[v + 1] - r0);
+      return true;
+    }
+    if ((uint32_t)seg + 1 >= g.nl) {
+      *p = g.lcol;
+      *len = 0;
+      return true;
+    }
+    const uint32_t* o = g.loff + (size_t)v * g.nl + seg;
+    const uint32_t b = o[0];
+    *p = g.lcol + r0 + b;
+    *len = o[1] - b;
+    return true;
+  }
+  const uint32_t pk = g.pkg_of[v];
+  const int32_t s = g.pkg_slot[pk];
+  if (s < 0) {
+    if (missing) *missing = pk;
+    const unsigned int bit = 1u << (pk & 31);
+    if (!(atomicOr(&g.miss_bits[pk >> 5], bit) & bit)) {
+      const unsigned long long i = atomicAdd(g.miss_count, 1ull);
+      if (i < g.miss_cap) g.miss_list[i] = pk;
+    }
+    return false;
+  }
+  g.used[s] = 1;
+  *p = g.cache + (uint64_t)s * g.slot_words + (r0 - g.pkg_base[pk]);
+  *len = (uint32_t)(g.rowptr[v + 1] - r0);
+  return true;
 }
 
 __device__ __forceinline__ unsigned long long wt_of(const KPlan& P, int u, uint32_t x) {
@@ -286,15 +375,22 @@ __device__ __forceinline__ uint32_t vertex_bound(const KPlan& P, const WaveState
   return ub;
 }
 
-__device__ void run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, uint32_t* arena,
+// The slot operations of depth d (vertex d just matched).  False when a row
+// is not resident (out of core); the caller defers.
+template <bool PG>
+__device__ bool run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, uint32_t* arena,
                         uint32_t cap, uint32_t lane, Counters& ct) {
   const uint32_t v = uni(ws.phi[d]);
   for (int o = P.op_begin[d]; o < P.op_end[d]; o++) {
     const KOp& op = P.ops[o];
-    const uint32_t* base;
-    uint32_t dv;
-    const uint64_t off = nbr_list(g, v, op.seg, &base, &dv);
-    const uint32_t* nv = uniptr(base + off);
+    const uint32_t* nv;
+    uint32_t dv, mp = 0;
+    const bool ok = rows<PG>(g, v, P.segs ? op.seg : -1, &nv, &dv, &mp);
+    if (!uni(ok)) {
+      ws.blk = uni(mp);
+      return false;
+    }
+    nv = uniptr(nv);
     dv = uni(dv);
     const uint32_t ub = P.labeled ? kInf : slot_bound(op, ws);
     const uint32_t nb = (ub == kInf) ? dv : uni(below(nv, dv, ub));
@@ -307,7 +403,7 @@ __device__ void run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, u
       if (ub != kInf) na = uni(below(a, na, ub));
       uint32_t* out = arena + (size_t)P.region[op.slot] * cap;
       uint32_t cnt;
-      if (P.labeled && op.seg < 0)
+      if (P.labeled && !(P.segs && op.seg >= 0))
         cnt = wave_isect<true, true, false>(a, na, nv, nb, op.labmask, g.lab, nullptr, out, lane,
                                             ct, nullptr);
       else
@@ -318,6 +414,7 @@ __device__ void run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, u
       ct.nops++;
     }
   }
+  return true;
 }
 
 // candidates of core vertex d+1 (truncated by its full symmetry bound)
@@ -347,20 +444,22 @@ __device__ __forceinline__ bool admit(const KGraph& g, const KPlan& P, const Wav
   return ok;
 }
 
-// next admitted candidate of core vertex d+1; false when exhausted
+// next admitted candidate of core vertex d+1 and its index; false when done
 __device__ __forceinline__ bool next_cand(const KGraph& g, const KPlan& P, WaveState& ws, int d,
-                                          uint32_t lane, uint32_t& x, Counters& ct) {
+                                          uint32_t lane, uint32_t& x, uint32_t& ix, Counters& ct) {
   for (;;) {
     uint64_t m = uni64(ws.cmask[d]);
     if (m) {
       const int i = __builtin_ctzll(m);
       ws.cmask[d] = m & (m - 1);
       x = uni(ws.cval[d][i]);
+      ix = uni(ws.cbase[d]) + i;
       return true;
     }
     const uint32_t base = uni(ws.cnext[d]);
     const uint32_t n = uni(ws.clen[d]);
     if (base >= n) return false;
+    ws.cbase[d] = base;
     ws.cnext[d] = base + kWave;
     const uint32_t* c = uniptr(ws.sptr[P.cand_slot[d]]);
     const uint32_t idx = base + lane;
@@ -377,29 +476,37 @@ __device__ __forceinline__ bool next_cand(const KGraph& g, const KPlan& P, WaveS
   }
 }
 
-// Weighted count of the last core vertex once 0..k-2 are fixed (one node).
+// Weighted count of the last core vertex once 0..k-2 are fixed (k == 3).
+// *ok = false when a row is not resident.
+template <bool PG>
 __device__ unsigned long long last_count(const KGraph& g, const KPlan& P, WaveState& ws,
-                                         uint32_t lane, Counters& ct) {
+                                         uint32_t lane, Counters& ct, bool* ok) {
   const int u = P.k - 1;
   const int dl = P.k - 2;
   const uint32_t ub = P.labeled ? kInf : vertex_bound(P, ws, u, dl);
   const unsigned long long* w2 = P.wt[u];
+  const bool segs = P.labeled && P.segs;
   const uint32_t* a;
   uint32_t na;
+  *ok = true;
   if (P.last_kind == 0) {
     a = uniptr(ws.sptr[P.last_src]);
     na = uni(ws.slen[P.last_src]);
   } else {
-    const uint32_t v = uni(ws.phi[dl]);
-    const uint32_t* base;
-    const uint64_t off = nbr_list(g, v, P.labeled ? (int)P.lab[u] : -1, &base, &na);
-    a = uniptr(base + off);
+    uint32_t mp = 0;
+    const bool r = rows<PG>(g, uni(ws.phi[dl]), segs ? (int)P.lab[u] : -1, &a, &na, &mp);
+    if (!uni(r)) {
+      ws.blk = uni(mp);
+      *ok = false;
+      return 0;
+    }
+    a = uniptr(a);
     na = uni(na);
   }
   if (ub != kInf) na = uni(below(a, na, ub));
   if (P.last_kind != 2) {
     if (P.labeled) {
-      const bool check = P.last_kind == 0 && P.last_check;
+      const bool check = (P.last_kind == 0 && P.last_check) || (P.last_kind == 1 && !segs);
       if (!check && !w2) return na;
       ct.nops++;
       return wave_label_sum(a, na, P.lab[u], check, g.lab, w2, lane, ct);
@@ -415,8 +522,12 @@ __device__ unsigned long long last_count(const KGraph& g, const KPlan& P, WaveSt
   ct.nops++;
   if (P.labeled) {
     unsigned long long s = 0;
-    uint32_t c = wave_isect<false, false, true>(p, np, a, na, 0, g.lab, w2, nullptr, lane, ct,
-                                                &s);
+    uint32_t c;
+    if (segs)
+      c = wave_isect<false, false, true>(p, np, a, na, 0, g.lab, w2, nullptr, lane, ct, &s);
+    else
+      c = wave_isect<false, true, true>(p, np, a, na, 1u << P.lab[u], g.lab, w2, nullptr, lane,
+                                        ct, &s);
     return w2 ? s : (unsigned long long)c;
   }
   uint32_t cnt = uni(wave_isect<false, false, false>(p, np, a, na, 0, g.lab, nullptr, nullptr,
@@ -431,11 +542,63 @@ __device__ unsigned long long last_count(const KGraph& g, const KPlan& P, WaveSt
   return cnt;
 }
 
+// Write one deferred piece (uniform call): depth and the candidate range
+// [a, b) at that depth below the current path.
+__device__ void emit_one(const DeferOut& O, const WaveState& ws, int depth, uint32_t a,
+                         uint32_t b, uint32_t blk, uint32_t lane) {
+  unsigned long long slot = 0;
+  if (lane == 0) slot = atomicAdd(O.count, 1ull);
+  slot = uni64(__shfl(slot, 0));
+  if (lane == 0 && slot < O.cap) {
+    Resume& R = O.buf[slot];
+    R.edge = ws.je;
+    R.r = ws.phi[0];
+    R.c = ws.phi[1];
+    R.lo = ws.jlo;
+    R.hi = ws.jhi;
+    R.depth = depth;
+    R.end = b;
+    for (int d = 1; d < depth && d < kMaxDepth; d++) R.idx[d] = ws.cidx[d];
+    if (depth >= 1 && depth < kMaxDepth) R.idx[depth] = a;
+    for (int d = 0; d <= depth && d < kMaxDepth; d++) R.phi[d] = ws.phi[d];
+    R.blk = blk;
+  }
+}
+
+// Lane-parallel emission: every lane with `mine` writes the piece "candidate
+// i at this depth" (range [i, i+1)), stopped by package blk.
+__device__ void emit_lanes(const DeferOut& O, const WaveState& ws, int depth, bool mine,
+                           uint32_t i, uint32_t blk, uint32_t lane) {
+  const uint64_t mm = __ballot(mine);
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(O.count, (unsigned long long)__popcll(mm));
+  base = uni64(__shfl(base, 0));
+  if (!mine) return;
+  const unsigned long long slot = base + __popcll(mm & ((1ull << lane) - 1ull));
+  if (slot >= O.cap) return;
+  Resume& R = O.buf[slot];
+  R.edge = ws.je;
+  R.r = ws.phi[0];
+  R.c = ws.phi[1];
+  R.lo = ws.jlo;
+  R.hi = ws.jhi;
+  R.depth = depth;
+  R.end = i + 1;
+  for (int d = 1; d < depth && d < kMaxDepth; d++) R.idx[d] = ws.cidx[d];
+  if (depth < kMaxDepth) R.idx[depth] = i;
+  for (int d = 0; d <= depth && d < kMaxDepth; d++) R.phi[d] = ws.phi[d];
+  R.blk = blk;
+}
+
 // The last two core vertices u1 = d+1, u2 = d+2 (d = k-3), fused: weighted
 // number of matches of (u1, u2) over the candidates of u1 in
-// [cnext[d], clen[d]).  Uniform result.
+// [cnext[d], clen[d]).  Batches of 64 candidates are all-or-nothing: when a
+// needed row is not resident, *fail = the batch's first index and the sum of
+// the finished batches is returned.  Uniform result.
+template <bool PG>
 __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveState& ws, int d,
-                                         uint32_t lane, Counters& ct) {
+                                         uint32_t lane, Counters& ct, const DeferOut& O,
+                                         bool* stopped) {
   const int u1 = d + 1, u2 = d + 2;
   const bool lab = P.labeled;
   const unsigned long long wd = uni64(ws.wacc[d]);
@@ -451,7 +614,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
   unsigned long long acc = 0, dropc = 0;
 
   if (P.last_kind == 0) {
-    // u2's set does not depend on u1's image
+    // u2's set does not depend on u1's image; no row of x is read
     const uint32_t* S = uniptr(ws.sptr[P.last_src]);
     const uint32_t nS = uni(ws.slen[P.last_src]);
     unsigned long long Lc = 0;
@@ -496,23 +659,49 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
     // u2's set is N(x): per-lane binary searches
     for (uint32_t base = cbeg; base < cend; base += kWave) {
       const uint32_t i = base + lane;
+      const uint32_t* nx = nullptr;
+      uint32_t dx = 0, x = 0, mp = 0;
+      bool ok = false, miss = false;
       if (i < cend) {
-        const uint32_t x = cand[i];
-        if (admit(g, P, ws, u1, P.cand_check[d], x)) {
-          const uint32_t* nx = g.colidx + g.rowptr[x];
-          const uint32_t dx = degree(g, x);
-          const uint32_t ub = sym_x ? min(ub_fixed, x) : ub_fixed;
-          const uint32_t c = below(nx, dx, ub);
-          uint32_t dr = 0, b = dropm;
-          while (b) {
-            int j = __builtin_ctz(b);
-            b &= b - 1;
-            const uint32_t y = ws.phi[j];
-            dr += (y < ub) && member(nx, c, y);
-          }
-          acc += c - dr;
-          ct.alg += 20ull;
+        x = cand[i];
+        ok = admit(g, P, ws, u1, P.cand_check[d], x);
+        if (ok) miss = !rows<PG>(g, x, -1, &nx, &dx, &mp);
+      }
+      bool stop_here = false;
+      uint32_t rest_from = 0;
+      const uint64_t mm = __ballot(miss);
+      if (mm) {
+        if (uni64(*(volatile unsigned long long*)O.count) > O.soft) {
+          // too many pieces already: finish the owners before the first
+          // miss; the rest of the range becomes one piece
+          const int f = __builtin_ctzll(mm);
+          ws.blk = uni((uint32_t)__shfl((int)mp, f));
+          rest_from = base + f;
+          stop_here = true;
+          if ((int)lane >= f) ok = false;
+        } else {
+          // owners whose row is not resident become deferred pieces
+          emit_lanes(O, ws, d, miss, i, mp, lane);
+          ok = ok && !miss;
         }
+      }
+      if (ok) {
+        const uint32_t ub = sym_x ? min(ub_fixed, x) : ub_fixed;
+        const uint32_t c = below(nx, dx, ub);
+        uint32_t dr = 0, b = dropm;
+        while (b) {
+          int j = __builtin_ctz(b);
+          b &= b - 1;
+          const uint32_t y = ws.phi[j];
+          dr += (y < ub) && member(nx, c, y);
+        }
+        acc += c - dr;
+        ct.alg += 20ull;
+      }
+      if (stop_here) {
+        emit_one(O, ws, d, rest_from, cend, uni(ws.blk), lane);
+        *stopped = true;
+        break;
       }
     }
     ct.nops++;
@@ -544,43 +733,61 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
     if (lane <= (uint32_t)d && ((dropm >> lane) & 1u)) in = member(Pset, nP, ws.phi[lane]);
     inP = (uint32_t)__ballot(in);
   }
-  const int seg2 = lab ? (int)P.lab[u2] : -1;  // labeled: u2's label part only
+  const bool segs = lab && P.segs;
+  const int seg2 = segs ? (int)P.lab[u2] : -1;  // labeled: u2's label part only
+  const bool check2 = lab && !segs;
   const bool pair = P.pair;
-  const uint32_t* lbase = lab ? g.lcol : g.colidx;
   const uint64_t hub_lim = isect ? (uint64_t)P.hub_factor * (uint64_t)max(nP, 1u) : ~0ull;
   for (uint32_t base = cbeg; base < cend; base += kWave) {
     const uint32_t i = base + lane;
     uint32_t x = 0, dx = 0, ub = ub_fixed;
     unsigned long long wx = 0, wb = 0;
-    uint64_t rp = 0;
-    bool ok = false, hub = false;
+    const uint32_t* nx = nullptr;
+    uint32_t mp = 0;
+    bool ok = false, hub = false, miss = false;
     if (i < cend) {
       x = cand[i];
       ok = admit(g, P, ws, u1, P.cand_check[d], x);
-      if (ok) {
-        const uint32_t* bb;
-        rp = nbr_list(g, x, seg2, &bb, &dx);
-        wx = wd * (w1 ? w1[x] : 1ull);
-        if (pair) {
-          // only partners y > x: the pair {x,y} is counted once, in both roles
-          const uint32_t s0 = below(lbase + rp, dx, x + 1);
-          rp += s0;
-          dx -= s0;
-          wb = wd * (w2 ? w2[x] : 1ull);
-        }
-        if (sym_x) ub = min(ub, x);
-        if (!lab) {
-          uint32_t b = inP;
-          const uint32_t* nx = lbase + rp;
-          while (b) {
-            int j = __builtin_ctz(b);
-            b &= b - 1;
-            const uint32_t y = ws.phi[j];
-            dropc += (y < ub) && member(nx, dx, y);
-          }
-        }
-        hub = (uint64_t)dx > hub_lim;
+      if (ok) miss = !rows<PG>(g, x, seg2, &nx, &dx, &mp);
+    }
+    bool stop_here = false;
+    uint32_t rest_from = 0;
+    const uint64_t mm = __ballot(miss);
+    if (mm) {
+      if (uni64(*(volatile unsigned long long*)O.count) > O.soft) {
+        // too many pieces already: finish the owners before the first miss;
+        // the rest of the range becomes one piece
+        const int f = __builtin_ctzll(mm);
+        ws.blk = uni((uint32_t)__shfl((int)mp, f));
+        rest_from = base + f;
+        stop_here = true;
+        if ((int)lane >= f) ok = false;
+      } else {
+        // owners whose row is not resident become deferred pieces
+        emit_lanes(O, ws, d, miss, i, mp, lane);
+        ok = ok && !miss;
       }
+    }
+    if (ok) {
+      wx = wd * (w1 ? w1[x] : 1ull);
+      if (pair) {
+        // only partners y > x: the pair {x,y} is counted once, in both roles
+        const uint32_t s0 = below(nx, dx, x + 1);
+        nx += s0;
+        dx -= s0;
+        wb = wd * (w2 ? w2[x] : 1ull);
+      }
+      if (sym_x) ub = min(ub, x);
+      if (!lab) {
+        uint32_t b = inP;
+        while (b) {
+          int j = __builtin_ctz(b);
+          b &= b - 1;
+          const uint32_t y = ws.phi[j];
+          dropc += (y < ub) && member(nx, dx, y);
+        }
+      }
+      hub = (uint64_t)dx > hub_lim;
     }
     // exclusive scan of the owners' list lengths
     const uint32_t len = (ok && !hub) ? dx : 0;
@@ -594,7 +801,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
     ws.own_ub[lane] = ub;
     ws.own_w[lane] = (ok && !hub) ? wx : 0ull;
     ws.own_b[lane] = (ok && !hub) ? wb : 0ull;
-    ws.own_rp[lane] = rp;
+    ws.own_ptr[lane] = (uint64_t)nx;
     __syncthreads();
     const uint32_t total = uni(ws.own_off[kWave]);
     ct.alg += 4ull * total + 16ull * kWave;
@@ -605,7 +812,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
         const uint32_t mid = (lo + hi) >> 1;
         if (ws.own_off[mid] <= t) lo = mid; else hi = mid;
       }
-      const uint32_t y = lbase[ws.own_rp[lo] + (t - ws.own_off[lo])];
+      const uint32_t y = ((const uint32_t*)ws.own_ptr[lo])[t - ws.own_off[lo]];
       if (y >= ws.own_ub[lo]) continue;
       bool hit = true;
       if (isect) {
@@ -618,6 +825,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
           hit = member(Pset, nP, y);
         }
       }
+      if (hit && check2) hit = g.lab[y] == P.lab[u2];
       if (hit) {
         acc += ws.own_w[lo] * (w2 ? w2[y] : 1ull);
         if (pair) acc += ws.own_b[lo] * (w1 ? w1[y] : 1ull);
@@ -634,29 +842,37 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
       const uint32_t ubo = uni((uint32_t)__shfl((int)ub, o));
       const unsigned long long wo = uni64(__shfl(wx, o));
       const unsigned long long wbo = uni64(__shfl(wb, o));
-      const uint32_t* nx = uniptr(lbase + uni64(__shfl(rp, o)));
+      const uint32_t* nxo = (const uint32_t*)uni64(ws.own_ptr[o]);
       uint32_t nxn = uni((uint32_t)__shfl((int)dx, o));
-      if (ubo != kInf) nxn = uni(below(nx, nxn, ubo));
+      if (ubo != kInf) nxn = uni(below(nxo, nxn, ubo));
       const uint32_t* Pp = Pset;
       uint32_t nPo = ubo != kInf ? uni(below(Pset, nP, ubo)) : nP;
-      if (pair) {  // partners y > x only (nx already starts past x)
+      if (pair) {  // partners y > x only (nxo already starts past x)
         const uint32_t s0 = uni(below(Pset, nPo, xo + 1));
         Pp = Pset + s0;
         nPo -= s0;
       }
       unsigned long long s = 0, s1 = 0;
       uint32_t c;
-      if (lab)
-        c = wave_isect<false, false, true>(Pp, nPo, nx, nxn, 0, g.lab, w2, nullptr, lane, ct, &s);
+      if (lab && !segs)
+        c = wave_isect<false, true, true>(Pp, nPo, nxo, nxn, 1u << P.lab[u2], g.lab, w2, nullptr,
+                                          lane, ct, &s);
+      else if (lab)
+        c = wave_isect<false, false, true>(Pp, nPo, nxo, nxn, 0, g.lab, w2, nullptr, lane, ct, &s);
       else
-        c = wave_isect<false, false, false>(Pp, nPo, nx, nxn, 0, g.lab, nullptr, nullptr, lane,
+        c = wave_isect<false, false, false>(Pp, nPo, nxo, nxn, 0, g.lab, nullptr, nullptr, lane,
                                             ct, nullptr);
       if (pair) {
         s1 = c;
-        if (w1) wave_isect<false, false, true>(Pp, nPo, nx, nxn, 0, g.lab, w1, nullptr, lane, ct,
+        if (w1) wave_isect<false, false, true>(Pp, nPo, nxo, nxn, 0, g.lab, w1, nullptr, lane, ct,
                                                &s1);
       }
       if (lane == 0) acc += wo * (lab && w2 ? s : (unsigned long long)c) + (pair ? wbo * s1 : 0ull);
+    }
+    if (stop_here) {
+      emit_one(O, ws, d, rest_from, cend, uni(ws.blk), lane);
+      *stopped = true;
+      break;
     }
   }
   ct.nops++;
@@ -664,22 +880,45 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
 }
 
 // DFS over one job (edge e = (r,c)) restricted to vertex-2 candidates
-// [lo, hi).  Phase 1 (may_split) may hand the tail of a long range to the
+// [lo, hi).  With `resume`, the deferred piece in ws.rdepth/ridx/rend: rebuild
+// its path and search only its candidate range at its depth.  Out of core, a
+// subtree that needs a non-resident row is written to O as a deferred piece
+// and the search goes on with the next candidate, so each piece is searched
+// exactly once.  Phase 1 (may_split) may hand a tail of the range to the
 // task buffer.
+template <bool PG>
 __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState& ws,
                                       uint32_t* arena, uint32_t cap, uint32_t lane, uint32_t r,
                                       uint32_t c, uint64_t e, uint32_t lo, uint32_t hi,
-                                      bool may_split, Task* tasks, unsigned long long* ntasks,
-                                      uint64_t task_cap, Counters& ct) {
+                                      bool resume, bool may_split, Task* tasks,
+                                      unsigned long long* ntasks, uint64_t task_cap,
+                                      const DeferOut& O, Counters& ct) {
   const int k = P.k;
+  ws.je = e;
+  ws.jlo = lo;
+  ws.jhi = hi;
   ws.phi[0] = r;
   ws.phi[1] = c;
   const unsigned long long w01 = uni64(wt_of(P, 0, r) * wt_of(P, 1, c));
   ws.wacc[1] = w01;
   if (k == 2) return w01;
-  run_ops(g, P, ws, 0, arena, cap, lane, ct);
-  if (k == 3) return w01 * last_count(g, P, ws, lane, ct);
-  run_ops(g, P, ws, 1, arena, cap, lane, ct);
+  if (!run_ops<PG>(g, P, ws, 0, arena, cap, lane, ct)) {
+    emit_one(O, ws, 0, 0, 0, uni(ws.blk), lane);
+    return 0;
+  }
+  if (k == 3) {
+    bool ok;
+    const unsigned long long v = last_count<PG>(g, P, ws, lane, ct, &ok);
+    if (!ok) {
+      emit_one(O, ws, 0, 0, 0, uni(ws.blk), lane);
+      return 0;
+    }
+    return w01 * v;
+  }
+  if (!run_ops<PG>(g, P, ws, 1, arena, cap, lane, ct)) {
+    emit_one(O, ws, 0, 0, 0, uni(ws.blk), lane);
+    return 0;
+  }
   setup_cand(P, ws, 1);
   const uint32_t n2 = uni(ws.clen[1]);
   const uint32_t b = min(lo, n2);
@@ -692,7 +931,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
     // wave keeps the first range and writes the others as tasks (hi = 0:
     // up to the next task's lo; hi = ~0: to the end of the list).
     const uint32_t* cand = uniptr(ws.sptr[P.cand_slot[1]]);
-    const int cseg = (P.labeled && P.k == 4) ? (int)P.lab[3] : -1;
+    const int cseg = (P.labeled && P.segs && k == 4) ? (int)P.lab[3] : -1;
     const uint64_t lt = (1ull << lane) - 1ull;
     uint64_t nstart = 0;
     uint32_t keep_end = en;
@@ -702,13 +941,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
       uint64_t run = 0, prev_c = 0, rank0 = 0;
       for (uint32_t base = b; base < en; base += kWave) {
         const uint32_t i = base + lane;
-        uint64_t cost = 0;
-        if (i < en) {
-          const uint32_t* bb;
-          uint32_t len;
-          nbr_list(g, cand[i], cseg, &bb, &len);
-          cost = 1ull + len;
-        }
+        const uint64_t cost = i < en ? 1ull + row_len(g, cand[i], cseg) : 0;
         uint64_t incl = cost;
         for (int o = 1; o < kWave; o <<= 1) {
           const uint64_t t = __shfl_up(incl, o);
@@ -761,25 +994,76 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
   }
   ws.cnext[1] = b;
   ws.clen[1] = en;
-  if (k == 4) return fused_tail(g, P, ws, 1, lane, ct);
+  ws.cmask[1] = 0;
+  int d = 1, stop = 1;  // stop: depth whose exhaustion ends this search
   unsigned long long cnt = 0;
-  int d = 1;
+  bool fused_now = (k == 4);  // the fused level is depth k-3
+  if (resume && uni(ws.rdepth) >= 1) {
+    // rebuild the path phi(2..D) from the recorded candidate indices
+    const int D = (int)uni(ws.rdepth);
+    for (int dd = 1; dd < D; dd++) ws.cidx[dd] = uni(ws.ridx[dd]);
+    for (int dd = 1; dd < D; dd++) {
+      const uint32_t ix = uni(ws.ridx[dd]);
+      const uint32_t x = uni(uniptr(ws.sptr[P.cand_slot[dd]])[ix]);
+      ws.phi[dd + 1] = x;
+      ws.wacc[dd + 1] = uni64(ws.wacc[dd]) * wt_of(P, dd + 1, x);
+      if (!run_ops<PG>(g, P, ws, dd + 1, arena, cap, lane, ct)) {
+        emit_one(O, ws, D, uni(ws.ridx[D]), uni(ws.rend), uni(ws.blk), lane);
+        return 0;
+      }
+      setup_cand(P, ws, dd + 1);
+    }
+    ws.cnext[D] = uni(ws.ridx[D]);
+    ws.clen[D] = min(uni(ws.clen[D]), uni(ws.rend));
+    ws.cmask[D] = 0;
+    d = D;
+    stop = D;
+    fused_now = (d == k - 3);
+  }
+  // the rest of the search above depth d: siblings after the current
+  // candidate at every depth stop..d-1, one piece each
+  auto emit_rest = [&](int upto) {
+    for (int dd = stop; dd < upto; dd++) {
+      const uint32_t a0 = uni(ws.cidx[dd]) + 1, b0 = uni(ws.clen[dd]);
+      if (a0 < b0) emit_one(O, ws, dd, a0, b0, uni(ws.blk), lane);
+    }
+  };
   for (;;) {
-    uint32_t x;
-    if (!next_cand(g, P, ws, d, lane, x, ct)) {
-      if (d == 1) break;
+    if (fused_now) {
+      fused_now = false;
+      bool stopped = false;
+      cnt += fused_tail<PG>(g, P, ws, d, lane, ct, O, &stopped);
+      if (stopped) {
+        emit_rest(d);
+        return cnt;
+      }
+      if (d == stop) break;
       d--;
       continue;
     }
+    uint32_t x, ix;
+    if (!next_cand(g, P, ws, d, lane, x, ix, ct)) {
+      if (d == stop) break;
+      d--;
+      continue;
+    }
+    ws.cidx[d] = ix;
     ws.phi[d + 1] = x;
     ws.wacc[d + 1] = uni64(ws.wacc[d]) * wt_of(P, d + 1, x);
-    run_ops(g, P, ws, d + 1, arena, cap, lane, ct);
+    if (!run_ops<PG>(g, P, ws, d + 1, arena, cap, lane, ct)) {
+      if (uni64(*(volatile unsigned long long*)O.count) > O.soft) {
+        // too many pieces already: stop, leaving the rest as a few pieces
+        emit_one(O, ws, d, ix, uni(ws.clen[d]), uni(ws.blk), lane);
+        emit_rest(d);
+        return cnt;
+      }
+      // the subtree of candidate ix waits for its row; go on with the rest
+      emit_one(O, ws, d, ix, ix + 1, uni(ws.blk), lane);
+      continue;
+    }
     setup_cand(P, ws, d + 1);
     d++;
-    if (d == k - 3) {
-      cnt += fused_tail(g, P, ws, d, lane, ct);
-      d--;
-    }
+    fused_now = (d == k - 3);
   }
   return cnt;
 }
@@ -808,19 +1092,33 @@ __device__ __forceinline__ uint32_t source_full(const KGraph& g, uint64_t e) {
 struct Launch {
   KGraph g;
   const KPlan* plan;
-  unsigned long long* cursor;  // [0] edge cursor, [1] task cursor, [2] task count
+  unsigned long long* cursor;  // [0] edges [1] task claim [2] tasks [3] resume claim [4] deferred
   uint64_t edge_lo, edge_hi;
   uint32_t chunk;
   uint32_t cap;
   uint32_t* arena;
   uint64_t arena_stride;
-  unsigned long long* out;       // count, jobs, alg bytes, set ops
+  unsigned long long* out;       // count, jobs, alg/ops edges, alg/ops tasks, alg fold
   unsigned long long* per_edge;  // optional, indexed e - edge_lo
   Task* tasks;
   uint64_t task_cap;
+  const Resume* rin;  // resume descriptors of this round
+  uint64_t nrin;
+  Resume* rout;  // deferred searches for the next round
+  uint64_t rout_cap;
 };
 
-__global__ __launch_bounds__(64, 6) void k_match_edges(Launch L) {
+__device__ __forceinline__ DeferOut defer_out(const Launch& L) {
+  DeferOut O;
+  O.buf = L.rout;
+  O.count = &L.cursor[4];
+  O.cap = L.rout_cap;
+  O.soft = L.rout_cap / 2;
+  return O;
+}
+
+template <bool PG>
+__global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
   const KPlan& P = *L.plan;
@@ -828,8 +1126,12 @@ __global__ __launch_bounds__(64, 6) void k_match_edges(Launch L) {
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
   unsigned long long total = 0, jobs = 0;
   Counters ct;
+  const DeferOut O = defer_out(L);
   const uint64_t span = L.edge_hi - L.edge_lo;
   for (;;) {
+    // out of core: stop taking new jobs once the piece buffer is half full;
+    // a later round continues the edge scan from the cursor
+    if (PG && uni64(*(volatile unsigned long long*)O.count) > O.soft) break;
     unsigned long long c0 = 0;
     if (lane == 0) c0 = atomicAdd(&L.cursor[0], (unsigned long long)L.chunk);
     c0 = uni64(__shfl(c0, 0));
@@ -843,7 +1145,7 @@ __global__ __launch_bounds__(64, 6) void k_match_edges(Launch L) {
       uint32_t r = rlo, c = 0;
       if (ok) {
         r = source_of(g, e, rlo);
-        c = g.colidx[e];
+        c = g.jcol[e];
         if (!P.labeled && P.rc_order && !(r > c)) ok = false;
         if (ok && P.labeled) ok = g.lab[r] == P.lab0 && g.lab[c] == P.lab1;
         if (ok) ok = degree(g, r) >= P.deg0 && degree(g, c) >= P.deg1 && g.maxnbr[c] >= P.deg2;
@@ -862,10 +1164,10 @@ __global__ __launch_bounds__(64, 6) void k_match_edges(Launch L) {
         jobs++;
         const uint32_t jr = uni(ws.jr[i]), jc = uni(ws.jc[i]);
         const unsigned long long cnt =
-            run_job(g, P, ws, arena, L.cap, lane, jr, jc, eb + i, 0, kInf, L.tasks != nullptr,
-                    L.tasks, &L.cursor[2], L.task_cap, ct);
+            run_job<PG>(g, P, ws, arena, L.cap, lane, jr, jc, eb + i, 0, kInf, false,
+                        L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
         total += cnt;
-        if (L.per_edge && lane == 0) atomicAdd(&L.per_edge[eb + i - L.edge_lo], cnt);
+        if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[eb + i - L.edge_lo], cnt);
       }
       __syncthreads();
     }
@@ -878,7 +1180,9 @@ __global__ __launch_bounds__(64, 6) void k_match_edges(Launch L) {
   }
 }
 
-__global__ __launch_bounds__(64, 6) void k_match_tasks(Launch L) {
+// Range tasks (phase 2), then -- in resume rounds -- resume descriptors.
+template <bool PG>
+__global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_tasks(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
   const KPlan& P = *L.plan;
@@ -886,8 +1190,11 @@ __global__ __launch_bounds__(64, 6) void k_match_tasks(Launch L) {
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
   unsigned long long total = 0;
   Counters ct;
-  const uint64_t ntask = min((uint64_t)L.cursor[2], L.task_cap);
+  const DeferOut O = defer_out(L);
+  const uint64_t ntask = L.tasks ? min((uint64_t)L.cursor[2], L.task_cap) : 0;
   for (;;) {
+    // out of core: likewise for tasks (claims then stay below the count)
+    if (PG && uni64(*(volatile unsigned long long*)O.count) > O.soft) break;
     unsigned long long t = 0;
     if (lane == 0) t = atomicAdd(&L.cursor[1], 1ull);
     t = uni64(__shfl(t, 0));
@@ -898,10 +1205,10 @@ __global__ __launch_bounds__(64, 6) void k_match_tasks(Launch L) {
     if (thi == 0) thi = uni(L.tasks[t + 1].lo);  // same job: slots are contiguous
     const uint32_t r = uni(L.tasks[t].r);
     const uint32_t c = uni(L.tasks[t].c);
-    const unsigned long long cnt =
-        run_job(g, P, ws, arena, L.cap, lane, r, c, e, tlo, thi, false, nullptr, nullptr, 0, ct);
+    const unsigned long long cnt = run_job<PG>(g, P, ws, arena, L.cap, lane, r, c, e, tlo, thi,
+                                               false, false, nullptr, nullptr, 0, O, ct);
     total += cnt;
-    if (L.per_edge && lane == 0) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
+    if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
   }
   if (lane == 0) {
     atomicAdd(&L.out[0], total);
@@ -910,11 +1217,49 @@ __global__ __launch_bounds__(64, 6) void k_match_tasks(Launch L) {
   }
 }
 
+// Resume round: every descriptor of the previous round, may split again.
+template <bool PG>
+__global__ __launch_bounds__(64, PG ? 4 : 6) void k_resume(Launch L) {
+  __shared__ WaveState ws;
+  const KGraph& g = L.g;
+  const KPlan& P = *L.plan;
+  const uint32_t lane = threadIdx.x;
+  uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
+  unsigned long long total = 0;
+  Counters ct;
+  const DeferOut O = defer_out(L);
+  for (;;) {
+    unsigned long long t = 0;
+    if (lane == 0) t = atomicAdd(&L.cursor[3], 1ull);
+    t = uni64(__shfl(t, 0));
+    if (t >= L.nrin) break;
+    const uint64_t e = uni64(L.rin[t].edge);
+    const uint32_t r = uni(L.rin[t].r), c = uni(L.rin[t].c);
+    const uint32_t rlo = uni(L.rin[t].lo), rhi = uni(L.rin[t].hi);
+    const int depth = (int)uni((uint32_t)L.rin[t].depth);
+    ws.rdepth = depth;
+    for (int d = 1; d <= depth && d < kMaxDepth; d++) ws.ridx[d] = uni(L.rin[t].idx[d]);
+    for (int d = 0; d <= depth && d < kMaxDepth; d++) ws.phi[d] = uni(L.rin[t].phi[d]);
+    ws.rend = uni(L.rin[t].end);
+    const unsigned long long cnt =
+        run_job<PG>(g, P, ws, arena, L.cap, lane, r, c, e, rlo, rhi, true,
+                    depth == 0 && L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
+    total += cnt;
+    if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
+  }
+  if (lane == 0) {
+    atomicAdd(&L.out[0], total);
+    atomicAdd(&L.out[2], (unsigned long long)ct.alg);
+    atomicAdd(&L.out[3], (unsigned long long)ct.nops);
+  }
+}
+
 // F(v) = sum over x in N(v) with lab[x] == want of prod_c child_c[x].
 // Edge-balanced: each wave walks 512 consecutive CSR edges per claim, 64 at
 // a time; lanes of the same row are summed with a segmented wave scan and
 // the last lane of each row segment adds into out[row] (zeroed before), so
-// a hub row is spread over many waves instead of serialising one.
+// a hub row is spread over many waves instead of serialising one.  Out of
+// core the rows are read from mapped host memory (streaming, coalesced).
 struct FoldArgs {
   KGraph g;
   uint32_t want;
@@ -940,7 +1285,7 @@ __global__ __launch_bounds__(64) void k_fold(FoldArgs A) {
       unsigned long long val = 0;
       if (e < c1) {
         row = source_of(g, e, r);
-        const uint32_t x = g.colidx[e];
+        const uint32_t x = g.jcol[e];
         if (g.lab[x] == A.want) {
           val = 1;
           for (int c = 0; c < A.nchild; c++) val *= A.child[c][x];
@@ -982,6 +1327,12 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     p[i] = v;
 }
 
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 #define HIP_OK(call)                                                \
   do {                                                              \
     hipError_t _e = (call);                                         \
@@ -995,6 +1346,7 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
 struct Workspace {
   int device = -1;
   hipStream_t stream = nullptr;
+  hipStream_t copy_stream = nullptr;  // host->HBM package copies
   void* arena = nullptr;
   size_t arena_bytes = 0;
   unsigned long long* cursor = nullptr;  // 8 words
@@ -1004,7 +1356,9 @@ struct Workspace {
   uint64_t task_cap = 0;
   void* tables = nullptr;  // fold / weight tables
   size_t table_bytes = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, eve = nullptr;
+  void* resume[2] = {nullptr, nullptr};
+  size_t resume_bytes[2] = {0, 0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, eve = nullptr, evc = nullptr;
   int cus = 0;
   int resident = 0;
 };
@@ -1015,6 +1369,7 @@ int ensure_workspace(int device) {
   if (g_ws.device == device) return PM_OK;
   HIP_OK(hipSetDevice(device));
   HIP_OK(hipStreamCreateWithFlags(&g_ws.stream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&g_ws.copy_stream, hipStreamNonBlocking));
   HIP_OK(hipMalloc(&g_ws.cursor, 8 * sizeof(unsigned long long)));
   HIP_OK(hipMalloc(&g_ws.out, 8 * sizeof(unsigned long long)));
   HIP_OK(hipMalloc(&g_ws.plan, sizeof(KPlan)));
@@ -1022,8 +1377,9 @@ int ensure_workspace(int device) {
   HIP_OK(hipEventCreate(&g_ws.ev1));
   HIP_OK(hipEventCreate(&g_ws.evf));
   HIP_OK(hipEventCreate(&g_ws.eve));
+  HIP_OK(hipEventCreate(&g_ws.evc));
   HIP_OK(hipDeviceGetAttribute(&g_ws.cus, hipDeviceAttributeMultiprocessorCount, device));
-  g_ws.task_cap = 1ull << 22;  // 4M range tasks (64 MiB)
+  g_ws.task_cap = 1ull << 22;  // 4M range tasks (96 MiB)
   HIP_OK(hipMalloc(&g_ws.tasks, sizeof(Task) * g_ws.task_cap));
   g_ws.device = device;
   return PM_OK;
@@ -1039,12 +1395,13 @@ int grow(void** p, size_t* have, size_t need) {
   return PM_OK;
 }
 
-void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
+void build_kplan(const Plan& p, bool segs, KPlan* kp, int* nregions) {
   std::memset(kp, 0, sizeof(*kp));
   const int kc = p.kc;
   kp->k = kc;
   kp->labeled = p.labeled;
   kp->rc_order = p.rc_order;
+  kp->segs = (p.labeled && segs) ? 1 : 0;
   kp->deg0 = p.deg[0];
   kp->deg1 = p.deg[1];
   kp->deg2 = p.deg[2];
@@ -1068,7 +1425,7 @@ void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
         ko.ncons = (int8_t)op.ncons;
         ko.labmask = op.labmask;
         for (int c = 0; c < op.ncons; c++) ko.cons_b[c] = op.cons_b[c];
-        ko.seg = (p.labeled && __builtin_popcount(op.labmask) == 1)
+        ko.seg = (kp->segs && __builtin_popcount(op.labmask) == 1)
                      ? (int8_t)__builtin_ctz(op.labmask)
                      : (int8_t)-1;
         if (op.kind == 1) kp->region[op.slot] = (int8_t)nreg++;
@@ -1078,12 +1435,14 @@ void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
   }
   kp->nops = nops;
   // a slot holds only label-`label` vertices when its label filter is that
-  // one label (INTERSECT filters, VIEW reads that label's part of the row)
+  // one label (INTERSECT filters; VIEW reads that label's part of the row,
+  // which needs the label-grouped rows)
   auto exact_slot = [&](int s, int label, int upto) {
     if (!p.labeled) return false;
     for (int dd = 0; dd <= upto; dd++)
       for (const SlotOp& op : p.ops[dd])
-        if (op.slot == s && op.labmask == (1u << label)) return true;
+        if (op.slot == s && op.labmask == (1u << label) && (op.kind == 1 || kp->segs))
+          return true;
     return false;
   };
   for (int d = 1; d <= kc - 3; d++) {
@@ -1101,15 +1460,16 @@ void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
       const SlotOp& op = p.ops[dl][0];
       kp->last_kind = op.kind == 0 ? 1 : 2;
       kp->last_src = op.parent;
-      kp->last_check = 0;  // read from the last vertex's label part of the row
+      kp->last_check = kp->segs ? 0 : 1;
     }
     // Labeled tail whose two vertices are interchangeable: u1's candidates
     // are the set P that u2 intersects with N(x), both label-exact with one
     // label.  Then the ordered pairs (x,y) are the adjacent pairs inside P,
     // and each unordered pair can be visited once (y > x) and credited in
     // both roles.
-    if (p.labeled && kc >= 4 && kp->last_kind == 2 && kp->last_src == kp->cand_slot[kc - 3] &&
-        p.clabel[kc - 2] == p.clabel[kc - 1] && kp->cand_check[kc - 3] == 0)
+    if (p.labeled && kp->segs && kc >= 4 && kp->last_kind == 2 &&
+        kp->last_src == kp->cand_slot[kc - 3] && p.clabel[kc - 2] == p.clabel[kc - 1] &&
+        kp->cand_check[kc - 3] == 0)
       kp->pair = 1;
   }
   *nregions = nreg;
@@ -1117,7 +1477,12 @@ void build_kplan(const Plan& p, KPlan* kp, int* nregions) {
 
 }  // namespace
 
-// (Re)build the label-grouped copy of the rows when the graph has labels.
+// ---------------------------------------------------------------------------
+// device graph
+// ---------------------------------------------------------------------------
+
+// (Re)build the label-grouped copy of the rows when the graph has labels and
+// its rows are resident.
 int upload_label_grouped(pm_graph* g) {
   DeviceGraph& d = g->d;
   if (d.lcol) HIP_OK(hipFree(d.lcol));
@@ -1126,7 +1491,7 @@ int upload_label_grouped(pm_graph* g) {
   d.loff = nullptr;
   d.nl = 0;
   const HostGraph& h = g->h;
-  if (h.labels.empty() || h.n == 0) return PM_OK;
+  if (h.labels.empty() || h.n == 0 || !d.colidx) return PM_OK;
   std::vector<uint32_t> lcol, loff;
   uint32_t nl = 0;
   build_label_grouped(h, &lcol, &loff, &nl);
@@ -1138,30 +1503,29 @@ int upload_label_grouped(pm_graph* g) {
   return PM_OK;
 }
 
-int device_upload(pm_graph* g) {
-  if (g->on_device) return PM_OK;
+// Per-vertex metadata always goes to HBM; the rows (colidx) only when
+// with_rows (out of core they stay in pinned host memory).
+int device_upload_rows(pm_graph* g, bool with_rows) {
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mu);
   int rc = ensure_workspace(dev);
   if (rc) return rc;
   const HostGraph& h = g->h;
-  std::vector<uint8_t> lab8(h.n, 0);
-  for (uint64_t v = 0; v < h.n && !h.labels.empty(); v++) lab8[v] = (uint8_t)h.labels[v];
-  std::vector<uint32_t> maxnbr;
-  compute_max_neighbour_degree(h, &maxnbr);
   DeviceGraph& d = g->d;
-  HIP_OK(hipMalloc(&d.rowptr, sizeof(uint64_t) * (h.n + 1)));
-  HIP_OK(hipMalloc(&d.colidx, sizeof(uint32_t) * std::max<uint64_t>(h.m, 1)));
-  HIP_OK(hipMalloc(&d.label8, std::max<uint64_t>(h.n, 1)));
-  HIP_OK(hipMalloc(&d.maxnbr, sizeof(uint32_t) * std::max<uint64_t>(h.n, 1)));
-  HIP_OK(hipMemcpy(d.rowptr, h.rowptr.data(), sizeof(uint64_t) * (h.n + 1), hipMemcpyHostToDevice));
-  if (h.m) HIP_OK(hipMemcpy(d.colidx, h.colidx.data(), sizeof(uint32_t) * h.m, hipMemcpyHostToDevice));
-  if (h.n) {
-    HIP_OK(hipMemcpy(d.label8, lab8.data(), h.n, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(d.maxnbr, maxnbr.data(), sizeof(uint32_t) * h.n, hipMemcpyHostToDevice));
-  }
-  {
+  if (!g->on_device) {
+    std::vector<uint8_t> lab8(h.n, 0);
+    for (uint64_t v = 0; v < h.n && !h.labels.empty(); v++) lab8[v] = (uint8_t)h.labels[v];
+    std::vector<uint32_t> maxnbr;
+    compute_max_neighbour_degree(h, &maxnbr);
+    HIP_OK(hipMalloc(&d.rowptr, sizeof(uint64_t) * (h.n + 1)));
+    HIP_OK(hipMalloc(&d.label8, std::max<uint64_t>(h.n, 1)));
+    HIP_OK(hipMalloc(&d.maxnbr, sizeof(uint32_t) * std::max<uint64_t>(h.n, 1)));
+    HIP_OK(hipMemcpy(d.rowptr, h.rowptr.data(), sizeof(uint64_t) * (h.n + 1), hipMemcpyHostToDevice));
+    if (h.n) {
+      HIP_OK(hipMemcpy(d.label8, lab8.data(), h.n, hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(d.maxnbr, maxnbr.data(), sizeof(uint32_t) * h.n, hipMemcpyHostToDevice));
+    }
     // source of every 64th edge: edge -> source lookups start here
     const uint64_t ns = (h.m + 63) / 64 + 1;
     std::vector<uint32_t> src64(ns, 0);
@@ -1172,14 +1536,22 @@ int device_upload(pm_graph* g) {
     }
     HIP_OK(hipMalloc(&d.src64, sizeof(uint32_t) * ns));
     HIP_OK(hipMemcpy(d.src64, src64.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+    d.bytes = sizeof(uint64_t) * (h.n + 1) + h.n + 4 * h.n + 4 * ns;
+    g->on_device = true;
+    g->device = dev;
   }
-  d.bytes = sizeof(uint64_t) * (h.n + 1) + sizeof(uint32_t) * h.m + h.n + 4 * h.n;
-  g->on_device = true;
-  rc = upload_label_grouped(g);
-  if (rc) return rc;
-  g->device = dev;
+  if (with_rows && !d.colidx) {
+    HIP_OK(hipMalloc(&d.colidx, sizeof(uint32_t) * std::max<uint64_t>(h.m, 1)));
+    if (h.m)
+      HIP_OK(hipMemcpy(d.colidx, h.colidx.data(), sizeof(uint32_t) * h.m, hipMemcpyHostToDevice));
+    d.bytes += sizeof(uint32_t) * h.m;
+    rc = upload_label_grouped(g);
+    if (rc) return rc;
+  }
   return PM_OK;
 }
+
+int device_upload(pm_graph* g) { return device_upload_rows(g, true); }
 
 int device_update_labels(pm_graph* g) {
   if (!g->on_device) return PM_OK;
@@ -1190,10 +1562,13 @@ int device_update_labels(pm_graph* g) {
   return upload_label_grouped(g);
 }
 
+void device_release_paging(pm_graph* g);
+
 void device_release(pm_graph* g) {
+  device_release_paging(g);
   if (!g->on_device) return;
   (void)hipFree(g->d.rowptr);
-  (void)hipFree(g->d.colidx);
+  if (g->d.colidx) (void)hipFree(g->d.colidx);
   (void)hipFree(g->d.label8);
   (void)hipFree(g->d.maxnbr);
   if (g->d.src64) (void)hipFree(g->d.src64);
@@ -1203,9 +1578,166 @@ void device_release(pm_graph* g) {
   g->on_device = false;
 }
 
-int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res) {
-  int rc = device_upload(g);
+// ---------------------------------------------------------------------------
+// out-of-core state of one graph
+// ---------------------------------------------------------------------------
+
+struct PagedGraph {
+  uint32_t slot_words = 0;
+  uint64_t nslots = 0;
+  PackageLayout layout;
+  uint32_t* host_rows = nullptr;  // pinned host copy of colidx
+  uint32_t* host_rows_dev = nullptr;  // its device address
+  int32_t* d_pkg_slot = nullptr;
+  uint32_t* d_pkg_of = nullptr;
+  uint64_t* d_pkg_base = nullptr;
+  uint32_t* d_cache = nullptr;
+  unsigned int* d_miss_bits = nullptr;
+  uint32_t* d_miss_list = nullptr;
+  unsigned long long* d_miss_count = nullptr;
+  uint32_t miss_cap = 0;
+  uint8_t* d_used = nullptr;
+  std::vector<int32_t> pkg_slot;  // host mirror
+  CachePolicy policy;
+};
+
+std::mutex g_paged_mu;
+std::vector<std::pair<const pm_graph*, std::unique_ptr<PagedGraph>>> g_paged;
+
+PagedGraph* find_paged(const pm_graph* g) {
+  for (auto& e : g_paged)
+    if (e.first == g) return e.second.get();
+  return nullptr;
+}
+
+void device_release_paging(pm_graph* g) {
+  std::lock_guard<std::mutex> lk(g_paged_mu);
+  for (size_t i = 0; i < g_paged.size(); i++) {
+    if (g_paged[i].first != g) continue;
+    PagedGraph* pg = g_paged[i].second.get();
+    if (pg->host_rows) (void)hipHostFree(pg->host_rows);
+    (void)hipFree(pg->d_pkg_slot);
+    (void)hipFree(pg->d_pkg_of);
+    (void)hipFree(pg->d_pkg_base);
+    (void)hipFree(pg->d_cache);
+    (void)hipFree(pg->d_miss_bits);
+    (void)hipFree(pg->d_miss_list);
+    (void)hipFree(pg->d_miss_count);
+    (void)hipFree(pg->d_used);
+    g_paged.erase(g_paged.begin() + i);
+    return;
+  }
+}
+
+// Set up (or reuse) the package cache of `cache_bytes` over slots of
+// `package_bytes`.
+int ensure_paged(pm_graph* g, uint64_t cache_bytes, uint32_t package_bytes, PagedGraph** out) {
+  const uint32_t words = std::max<uint32_t>(package_bytes, 4096) / 4;
+  const uint64_t nslots = cache_bytes / ((uint64_t)words * 4);
+  std::lock_guard<std::mutex> lk(g_paged_mu);
+  PagedGraph* pg = find_paged(g);
+  if (pg && pg->slot_words == words && pg->nslots == nslots) {
+    *out = pg;
+    return PM_OK;
+  }
+  if (pg) {
+    // different geometry: rebuild
+    for (size_t i = 0; i < g_paged.size(); i++)
+      if (g_paged[i].first == g) {
+        PagedGraph* q = g_paged[i].second.get();
+        if (q->host_rows) (void)hipHostFree(q->host_rows);
+        (void)hipFree(q->d_pkg_slot);
+        (void)hipFree(q->d_pkg_of);
+        (void)hipFree(q->d_pkg_base);
+        (void)hipFree(q->d_cache);
+        (void)hipFree(q->d_miss_bits);
+        (void)hipFree(q->d_miss_list);
+        (void)hipFree(q->d_miss_count);
+        (void)hipFree(q->d_used);
+        g_paged.erase(g_paged.begin() + i);
+        break;
+      }
+  }
+  const HostGraph& h = g->h;
+  auto npg = std::make_unique<PagedGraph>();
+  pg = npg.get();
+  pg->slot_words = words;
+  pg->nslots = nslots;
+  int rc = build_packages(h, words, &pg->layout);
   if (rc) return rc;
+  uint32_t widest = 1;
+  for (uint32_t w : pg->layout.nslots) widest = std::max(widest, w);
+  if (nslots < (uint64_t)widest + 72) {
+    set_error("out-of-core cache too small: it must hold the widest package plus 72 slots");
+    return PM_ERR_ARG;
+  }
+  const uint32_t np = pg->layout.count();
+  HIP_OK(hipHostMalloc((void**)&pg->host_rows, sizeof(uint32_t) * std::max<uint64_t>(h.m, 1),
+                       hipHostMallocMapped));
+  if (h.m) std::memcpy(pg->host_rows, h.colidx.data(), sizeof(uint32_t) * h.m);
+  HIP_OK(hipHostGetDevicePointer((void**)&pg->host_rows_dev, pg->host_rows, 0));
+  HIP_OK(hipMalloc(&pg->d_pkg_slot, sizeof(int32_t) * std::max(np, 1u)));
+  HIP_OK(hipMalloc(&pg->d_pkg_of, sizeof(uint32_t) * std::max<uint64_t>(h.n, 1)));
+  HIP_OK(hipMalloc(&pg->d_pkg_base, sizeof(uint64_t) * std::max(np, 1u)));
+  HIP_OK(hipMalloc(&pg->d_cache, (size_t)nslots * words * 4));
+  pg->miss_cap = std::max(np, 1u);
+  HIP_OK(hipMalloc(&pg->d_miss_bits, sizeof(unsigned int) * ((np + 31) / 32 + 1)));
+  HIP_OK(hipMalloc(&pg->d_miss_list, sizeof(uint32_t) * pg->miss_cap));
+  HIP_OK(hipMalloc(&pg->d_miss_count, sizeof(unsigned long long)));
+  HIP_OK(hipMalloc(&pg->d_used, nslots));
+  if (h.n)
+    HIP_OK(hipMemcpy(pg->d_pkg_of, pg->layout.pkg_of.data(), sizeof(uint32_t) * h.n,
+                     hipMemcpyHostToDevice));
+  if (np)
+    HIP_OK(hipMemcpy(pg->d_pkg_base, pg->layout.edge_base.data(), sizeof(uint64_t) * np,
+                     hipMemcpyHostToDevice));
+  g_paged.emplace_back(g, std::move(npg));
+  *out = pg;
+  return PM_OK;
+}
+
+// Load `want` (package ids) into victims chosen by the policy; packages in
+// `keep` are not evicted.  Copies go on the copy stream; returns bytes.
+int load_packages(PagedGraph* pg, const std::vector<uint32_t>& want, std::vector<uint8_t>& pinned,
+                  hipStream_t cs, uint64_t* bytes) {
+  *bytes = 0;
+  const PackageLayout& L = pg->layout;
+  for (uint32_t p : want) {
+    if (pg->pkg_slot[p] >= 0) continue;
+    const uint32_t width = L.nslots[p];
+    const int64_t s = pg->policy.victim(width, pinned);
+    if (s < 0) break;  // everything left is pinned: the rest waits a round
+    for (uint32_t w = 0; w < width; w++) {
+      const int32_t old = pg->policy.owner((uint32_t)s + w);
+      if (old >= 0) {
+        // evict the whole package that owned the slot
+        const int32_t os = pg->pkg_slot[old];
+        for (uint32_t k = 0; k < L.nslots[old]; k++) pg->policy.release((uint32_t)os + k);
+        pg->pkg_slot[old] = -1;
+      }
+    }
+    pg->policy.assign((uint32_t)s, width, (int32_t)p);
+    for (uint32_t w = 0; w < width; w++) pinned[s + w] = 1;
+    pg->pkg_slot[p] = (int32_t)s;
+    const uint64_t nbytes = L.words[p] * 4;
+    if (nbytes)
+      HIP_OK(hipMemcpyAsync(pg->d_cache + (uint64_t)s * pg->slot_words,
+                            pg->host_rows + L.edge_base[p], nbytes, hipMemcpyHostToDevice, cs));
+    *bytes += nbytes;
+  }
+  return PM_OK;
+}
+
+int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res) {
+  const bool paged = opts->cache_bytes > 0;
+  int rc = device_upload_rows(g, !paged);
+  if (rc) return rc;
+  PagedGraph* pg = nullptr;
+  if (paged) {
+    rc = ensure_paged(g, opts->cache_bytes, opts->package_bytes ? opts->package_bytes : (2u << 20),
+                      &pg);
+    if (rc) return rc;
+  }
   std::lock_guard<std::mutex> lk(g_ws_mu);
   rc = ensure_workspace(g->device);
   if (rc) return rc;
@@ -1216,12 +1748,12 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   }
   KPlan kp;
   int nreg = 0;
-  build_kplan(plan, &kp, &nreg);
+  build_kplan(plan, !paged && g->d.nl > 0, &kp, &nreg);
   hipStream_t st = opts->stream ? (hipStream_t)opts->stream : g_ws.stream;
   // default: every wave the chip can hold at once (persistent grid)
   if (g_ws.resident == 0) {
     int per_cu = 0;
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_match_edges, kWave, 0));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_match_edges<false>, kWave, 0));
     g_ws.resident = std::max(1, per_cu) * g_ws.cus;
   }
   const int waves = opts->waves > 0 ? opts->waves : g_ws.resident;
@@ -1257,13 +1789,43 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   const uint64_t stride = (uint64_t)std::max(nreg, 1) * cap;
   rc = grow(&g_ws.arena, &g_ws.arena_bytes, (size_t)waves * stride * sizeof(uint32_t));
   if (rc) return rc;
+  const uint64_t rcap = paged ? (1ull << 22) : 1;  // deferred pieces per round
+  for (int i = 0; i < 2; i++) {
+    rc = grow(&g_ws.resume[i], &g_ws.resume_bytes[i], sizeof(Resume) * rcap);
+    if (rc) return rc;
+  }
   HIP_OK(hipMemcpyAsync(g_ws.plan, &kp, sizeof(kp), hipMemcpyHostToDevice, st));
   unsigned long long* d_per_edge = nullptr;
   if (opts->per_edge && hi > lo)
     HIP_OK(hipMallocAsync((void**)&d_per_edge, sizeof(unsigned long long) * (hi - lo), st));
-  const KGraph kg{h.n,         h.m,       g->d.rowptr, g->d.colidx, g->d.label8, g->d.maxnbr,
-                  g->d.lcol,   g->d.loff, g->d.nl,     g->d.src64};
+
+  KGraph kg;
+  std::memset(&kg, 0, sizeof(kg));
+  kg.n = h.n;
+  kg.m = h.m;
+  kg.rowptr = g->d.rowptr;
+  kg.colidx = g->d.colidx;
+  kg.jcol = paged ? pg->host_rows_dev : g->d.colidx;
+  kg.lab = g->d.label8;
+  kg.maxnbr = g->d.maxnbr;
+  kg.lcol = g->d.lcol;
+  kg.loff = g->d.loff;
+  kg.nl = paged ? 0 : g->d.nl;
+  kg.src64 = g->d.src64;
+  if (paged) {
+    kg.pkg_slot = pg->d_pkg_slot;
+    kg.pkg_of = pg->d_pkg_of;
+    kg.pkg_base = pg->d_pkg_base;
+    kg.cache = pg->d_cache;
+    kg.slot_words = pg->slot_words;
+    kg.miss_bits = pg->d_miss_bits;
+    kg.miss_list = pg->d_miss_list;
+    kg.miss_count = pg->d_miss_count;
+    kg.miss_cap = pg->miss_cap;
+    kg.used = pg->d_used;
+  }
   Launch L;
+  std::memset(&L, 0, sizeof(L));
   L.g = kg;
   L.plan = g_ws.plan;
   L.cursor = g_ws.cursor;
@@ -1276,16 +1838,51 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   L.out = g_ws.out;
   L.tasks = (plan.kc >= 4 && opts->split >= 0) ? g_ws.tasks : nullptr;
   L.task_cap = g_ws.task_cap;
+  L.rout_cap = rcap;
   const int reps = opts->repeats > 0 ? opts->repeats : 1;
-  double sum_ms = 0, min_ms = 1e30, phase[3] = {0, 0, 0};
+  double sum_ms = 0, min_ms = 1e30, phase[3] = {0, 0, 0}, link_ms = 0;
+  uint64_t link_bytes = 0, misses = 0, deferred = 0;
+  int rounds = 1;
   const int fold_blocks = (int)std::max<uint64_t>(
       1, std::min<uint64_t>((h.m + kFoldChunk - 1) / kFoldChunk, (uint64_t)g_ws.cus * 16));
+  const uint32_t np = paged ? pg->layout.count() : 0;
   for (int r = 0; r < reps; r++) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor, 8, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.out, 8, 0ull);
     L.per_edge = (r == 0) ? d_per_edge : nullptr;
     if (L.per_edge)
       hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, st, d_per_edge, hi - lo, 0ull);
+    const auto t0 = std::chrono::steady_clock::now();
+    link_bytes = misses = deferred = 0;
+    link_ms = 0;
+    rounds = 1;
+    if (paged) {
+      // empty cache, then the reference's initial preload (buffer_init_on_gpu,
+      // src/buffer.cu:264-415): packages in id order until the cache is full
+      pg->policy.reset((uint32_t)pg->nslots, opts->policy);
+      pg->pkg_slot.assign(np, -1);
+      std::vector<uint8_t> pinned(pg->nslots, 0);
+      std::vector<uint32_t> pre;
+      uint64_t used_slots = 0;
+      for (uint32_t p = 0; p < np && used_slots + pg->layout.nslots[p] <= pg->nslots; p++) {
+        pre.push_back(p);
+        used_slots += pg->layout.nslots[p];
+      }
+      HIP_OK(hipEventRecord(g_ws.evc, g_ws.copy_stream));
+      uint64_t b = 0;
+      rc = load_packages(pg, pre, pinned, g_ws.copy_stream, &b);
+      if (rc) return rc;
+      link_bytes += b;
+      HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
+                            hipMemcpyHostToDevice, g_ws.copy_stream));
+      HIP_OK(hipEventRecord(g_ws.evc, g_ws.copy_stream));
+      HIP_OK(hipStreamWaitEvent(st, g_ws.evc, 0));
+      hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_miss_bits,
+                         (uint64_t)((np + 31) / 32 + 1), 0u);
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, pg->d_miss_count, 1, 0ull);
+      hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_used,
+                         (pg->nslots + 3) / 4, 0u);
+    }
     HIP_OK(hipEventRecord(g_ws.ev0, st));
     // pendant-tree tables, children first
     for (int i = 0; i < plan.nfold; i++) {
@@ -1315,9 +1912,10 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
       if (h.n) hipLaunchKernelGGL(k_product, dim3(1024), dim3(256), 0, st, h.n, A);
     }
     HIP_OK(hipEventRecord(g_ws.evf, st));
-    hipLaunchKernelGGL(k_match_edges, dim3(waves), dim3(kWave), 0, st, L);
+    L.rout = (Resume*)g_ws.resume[1];
+    hipLaunchKernelGGL(paged ? k_match_edges<true> : k_match_edges<false>, dim3(waves), dim3(kWave), 0, st, L);
     HIP_OK(hipEventRecord(g_ws.eve, st));
-    if (L.tasks) hipLaunchKernelGGL(k_match_tasks, dim3(waves), dim3(kWave), 0, st, L);
+    if (L.tasks) hipLaunchKernelGGL(paged ? k_match_tasks<true> : k_match_tasks<false>, dim3(waves), dim3(kWave), 0, st, L);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(g_ws.ev1, st));
     HIP_OK(hipEventSynchronize(g_ws.ev1));
@@ -1329,8 +1927,139 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
     phase[0] = mf;
     phase[1] = me;
     phase[2] = mt;
-    sum_ms += ms;
-    min_ms = std::min(min_ms, (double)ms);
+    if (paged) {
+      // Rounds.  Deferred searches wait on the host; each round admits them
+      // oldest first while the union of what they need -- the packages of
+      // their path rows and of the package that stopped them -- fits the
+      // cache, loads exactly that set (pinned against eviction) and resumes
+      // only those.  Every admitted search then passes its stopping point,
+      // so the rounds terminate; nothing is dropped, so counts are exact.
+      const PackageLayout& PL = pg->layout;
+      const uint64_t span = hi - lo;
+      const size_t admit_cap = 1u << 18;  // pieces resumed per round
+      std::vector<Resume> pending, fresh;
+      for (;;) {
+        unsigned long long cur8[8], mc = 0;
+        HIP_OK(hipMemcpy(cur8, g_ws.cursor, sizeof(cur8), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(&mc, pg->d_miss_count, sizeof(mc), hipMemcpyDeviceToHost));
+        const uint64_t nd = cur8[4];
+        if (nd > rcap) {
+          set_error("out-of-core: more deferred pieces in one round than the buffer holds");
+          return PM_ERR_OVERFLOW;
+        }
+        if (cur8[2] > g_ws.task_cap) {
+          set_error("out-of-core: range task buffer overflow");
+          return PM_ERR_OVERFLOW;
+        }
+        misses += mc;
+        deferred += nd;
+        if (nd) {
+          fresh.resize(nd);
+          HIP_OK(hipMemcpy(fresh.data(), g_ws.resume[1], sizeof(Resume) * nd,
+                           hipMemcpyDeviceToHost));
+          pending.insert(pending.end(), fresh.begin(), fresh.end());
+        }
+        const bool edges_left = cur8[0] < span;
+        const uint64_t tasks_done = std::min(cur8[1], cur8[2]);
+        const bool tasks_left = L.tasks && tasks_done < cur8[2];
+        if (pending.empty() && !edges_left && !tasks_left) break;
+        std::vector<uint8_t> used(pg->nslots);
+        HIP_OK(hipMemcpy(used.data(), pg->d_used, pg->nslots, hipMemcpyDeviceToHost));
+        pg->policy.touched(used.data(), (uint32_t)pg->nslots);
+        // admission: oldest pieces first while their needs fit the cache
+        auto needs = [&](const Resume& d, uint32_t* need) {
+          int nneed = 0;
+          for (int j = 0; j <= d.depth && j < kMaxDepth; j++) need[nneed++] = PL.pkg_of[d.phi[j]];
+          need[nneed++] = d.blk;
+          return nneed;
+        };
+        std::vector<uint8_t> in_set(np, 0);
+        std::vector<uint32_t> want;
+        uint64_t set_slots = 0;
+        size_t nadmit = 0;
+        for (const Resume& d : pending) {
+          if (nadmit >= admit_cap) break;
+          uint32_t need[kMaxDepth + 1];
+          const int nneed = needs(d, need);
+          uint64_t extra = 0;
+          for (int q = 0; q < nneed; q++) {
+            bool dup = in_set[need[q]] != 0;
+            for (int z = 0; z < q && !dup; z++) dup = need[z] == need[q];
+            if (!dup) extra += PL.nslots[need[q]];
+          }
+          if (set_slots + extra > pg->nslots) break;
+          for (int q = 0; q < nneed; q++)
+            if (!in_set[need[q]]) {
+              in_set[need[q]] = 1;
+              want.push_back(need[q]);
+            }
+          set_slots += extra;
+          nadmit++;
+        }
+        if (nadmit == 0 && !pending.empty() && !edges_left && !tasks_left) {
+          set_error("out-of-core: a deferred search needs more packages than the cache holds");
+          return PM_ERR_OVERFLOW;
+        }
+        std::vector<uint8_t> pinned(pg->nslots, 0);
+        for (uint32_t q : want)
+          if (pg->pkg_slot[q] >= 0)
+            for (uint32_t w = 0; w < PL.nslots[q]; w++) pinned[pg->pkg_slot[q] + w] = 1;
+        const auto c0 = std::chrono::steady_clock::now();
+        uint64_t b = 0;
+        rc = load_packages(pg, want, pinned, g_ws.copy_stream, &b);
+        if (rc) return rc;
+        link_bytes += b;
+        // resume every piece whose needs are now resident (the admitted ones
+        // and any other that finds its packages in the cache)
+        std::vector<Resume> admitted, rest;
+        for (const Resume& d : pending) {
+          uint32_t need[kMaxDepth + 1];
+          const int nneed = needs(d, need);
+          bool here = admitted.size() < admit_cap;
+          for (int q = 0; q < nneed && here; q++) here = pg->pkg_slot[need[q]] >= 0;
+          (here ? admitted : rest).push_back(d);
+        }
+        HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
+                              hipMemcpyHostToDevice, g_ws.copy_stream));
+        if (!admitted.empty())
+          HIP_OK(hipMemcpyAsync(g_ws.resume[0], admitted.data(), sizeof(Resume) * admitted.size(),
+                                hipMemcpyHostToDevice, g_ws.copy_stream));
+        HIP_OK(hipStreamSynchronize(g_ws.copy_stream));
+        link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
+                       .count();
+        pending.swap(rest);
+        // next round: continue the edge scan and the task queue, resume the
+        // admitted pieces (they may split and defer again)
+        unsigned long long nc[5] = {cur8[0], tasks_left ? tasks_done : 0ull,
+                                    tasks_left ? cur8[2] : 0ull, 0ull, 0ull};
+        HIP_OK(hipMemcpyAsync(g_ws.cursor, nc, sizeof(nc), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_miss_bits,
+                           (uint64_t)((np + 31) / 32 + 1), 0u);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, pg->d_miss_count, 1, 0ull);
+        hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_used,
+                           (pg->nslots + 3) / 4, 0u);
+        L.rin = (const Resume*)g_ws.resume[0];
+        L.nrin = admitted.size();
+        L.rout = (Resume*)g_ws.resume[1];
+        if (edges_left) hipLaunchKernelGGL(k_match_edges<true>, dim3(waves), dim3(kWave), 0, st, L);
+        if (L.nrin) hipLaunchKernelGGL(k_resume<true>, dim3(waves), dim3(kWave), 0, st, L);
+        if (L.tasks) hipLaunchKernelGGL(k_match_tasks<true>, dim3(waves), dim3(kWave), 0, st, L);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(st));
+        rounds++;
+        if (rounds > 1000000) {
+          set_error("out-of-core: no progress");
+          return PM_ERR_OVERFLOW;
+        }
+      }
+      L.rin = nullptr;
+      L.nrin = 0;
+    }
+    const double wall =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const double t = paged ? wall : (double)ms;
+    sum_ms += t;
+    min_ms = std::min(min_ms, t);
   }
   unsigned long long out[8], cur[8];
   HIP_OK(hipMemcpyAsync(out, g_ws.out, sizeof(out), hipMemcpyDeviceToHost, st));
@@ -1341,6 +2070,10 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
     HIP_OK(hipFreeAsync(d_per_edge, st));
   }
   HIP_OK(hipStreamSynchronize(st));
+  if (!paged && cur[4] != 0) {
+    set_error("internal: a resident-row run deferred work");
+    return PM_ERR_DEVICE;
+  }
   uint64_t total = out[0];
   if (!plan.labeled) total *= (uint64_t)plan.aut;  // test.cu:198
   std::memset(res, 0, sizeof(*res));
@@ -1357,6 +2090,12 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   res->tasks = std::min<uint64_t>(cur[2], g_ws.task_cap);
   res->waves = waves;
   res->chunk = (int32_t)chunk;
+  res->link_bytes = link_bytes;
+  res->link_ms = link_ms;
+  res->misses = misses;
+  res->deferred = deferred;
+  res->rounds = rounds;
+  res->packages = (int32_t)np;
   return PM_OK;
 }
 
