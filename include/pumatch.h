@@ -68,6 +68,9 @@ typedef struct {
                            src/buffer.cu:264-415,800-929 */
   uint32_t package_bytes; /* cache slot / package size (0 = 2 MiB) */
   int32_t prefetch;     /* out-of-core: load predicted packages before each round */
+  int32_t order;        /* labeled patterns: 0 = cost-based order of pattern
+                           vertices 2.. (same count, less work), 1 = file
+                           order as the reference (src/pattern.h:67-100) */
 } pm_opts;
 
 typedef struct {
@@ -75,7 +78,8 @@ typedef struct {
   double min_ms;        /* fastest of the timed launches */
   uint64_t count;       /* match count (x|Aut| when unlabeled, test.cu:198) */
   uint64_t jobs;        /* jobs that passed the filter (src/job_queue.h:44-61) */
-  uint64_t alg_bytes;   /* algorithmic bytes of one launch (see DESIGN) */
+  uint64_t alg_bytes;   /* algorithmic bytes of one launch: list entries
+                           read and written x 4, job and table entries */
   uint64_t set_ops;     /* set operations executed by one launch */
   uint64_t link_bytes;  /* host->HBM bytes paged during the run (out-of-core) */
   uint64_t tasks;       /* range tasks created by heavy-root splitting */
@@ -172,7 +176,13 @@ void pm_opts_default(pm_opts* o);
  * inside the launch). */
 int pm_match(pm_graph* g, const pm_pattern* p, const pm_opts* opts, pm_result* res);
 
-/* Run n patterns back to back on one graph; results[i] as from pm_match. */
+/* Count n patterns of 2..PM_MAX_PATTERN vertices, labeled and unlabeled
+ * mixed, in ONE persistent launch: the job space is (pattern, edge chunk) pairs on
+ * one cursor, range tasks and deferred pieces carry their pattern, and equal
+ * pendant subtrees share one table.  results[i].count/jobs are pattern i's
+ * and equal a separate pm_match of p[i]; the timing and traffic fields of
+ * every result describe the whole launch.  Replaces running test.cu once per
+ * pattern file (plot/plot_fig11.sh:112-123).  opts->per_edge needs n == 1. */
 int pm_match_batch(pm_graph* g, const pm_pattern* const* p, int32_t n, const pm_opts* opts,
                    pm_result* results);
 

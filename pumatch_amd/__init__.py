@@ -24,6 +24,8 @@ CLI_PATH = os.path.join(_HERE, "bin", "test")
 
 PM_GRAPH_LABELS = 0x1
 PM_GRAPH_HOST_ONLY = 0x2
+PM_OK, PM_ERR_IO, PM_ERR_FORMAT, PM_ERR_PATTERN = 0, -1, -2, -3
+PM_ERR_DEVICE, PM_ERR_ARG, PM_ERR_OVERFLOW, PM_ERR_NOMEM = -4, -5, -6, -7
 MAX_PATTERN = 7
 
 
@@ -46,6 +48,7 @@ class Opts(C.Structure):
         ("cache_bytes", C.c_uint64),
         ("package_bytes", C.c_uint32),
         ("prefetch", C.c_int32),
+        ("order", C.c_int32),
     ]
 
 
@@ -303,14 +306,12 @@ class MatchResult:
     per_edge: np.ndarray | None = None
 
 
+ORDER = {"cost": 0, "file": 1}
 POLICY = {"lru": 0, "fifo": 1, "learned": 2}
 
 
-def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int = 1,
-          split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
-          per_edge: bool = False, stream: int | None = None, cache_bytes: int = 0,
-          package_bytes: int = 0, policy: str = "lru", prefetch: bool = False) -> MatchResult:
-    """Count matches of p in g on the GPU (pm_match)."""
+def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
+               stream, cache_bytes, package_bytes, policy, prefetch, order="cost"):
     o = Opts()
     lib().pm_opts_default(C.byref(o))
     o.waves, o.chunk, o.repeats = waves, chunk, repeats
@@ -318,19 +319,54 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
     o.cache_bytes, o.package_bytes = cache_bytes, package_bytes
     o.policy, o.prefetch = POLICY[policy], 1 if prefetch else 0
     o.edge_lo, o.edge_hi = edge_lo, edge_hi
+    o.order = ORDER[order]
     pe = None
     if per_edge:
         pe = np.zeros(g.m, dtype=np.uint64)
         o.per_edge = pe.ctypes.data_as(C.POINTER(C.c_uint64))
     if stream is not None:
         o.stream = stream
-    r = Result()
-    _check(lib().pm_match(g._h, p._h, C.byref(o), C.byref(r)), "match")
+    return o, pe
+
+
+def _result(r: Result, pe=None) -> MatchResult:
     return MatchResult(r.kernel_ms, r.min_ms, int(r.count), int(r.jobs), int(r.alg_bytes),
                        int(r.set_ops), int(r.link_bytes), int(r.tasks), int(r.waves), int(r.chunk),
                        tuple(r.phase_ms), tuple(int(x) for x in r.phase_alg_bytes),
                        int(r.misses), int(r.deferred), int(r.rounds), int(r.packages),
                        float(r.link_ms), pe)
+
+
+def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int = 1,
+          split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
+          per_edge: bool = False, stream: int | None = None, cache_bytes: int = 0,
+          package_bytes: int = 0, policy: str = "lru", prefetch: bool = False,
+          order: str = "cost") -> MatchResult:
+    """Count matches of p in g on the GPU (pm_match).  order="file" matches
+    labeled patterns in file order as the reference does; "cost" (default)
+    lets the planner pick the order of vertices 2.. (same count)."""
+    o, pe = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
+                       stream, cache_bytes, package_bytes, policy, prefetch, order)
+    r = Result()
+    _check(lib().pm_match(g._h, p._h, C.byref(o), C.byref(r)), "match")
+    return _result(r, pe)
+
+
+def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: int = 1,
+                split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
+                stream: int | None = None, cache_bytes: int = 0, package_bytes: int = 0,
+                policy: str = "lru", prefetch: bool = False, order: str = "cost") -> list:
+    """Count several patterns in one persistent launch (pm_match_batch).
+    Returns one MatchResult per pattern; count/jobs are the pattern's own,
+    the timing fields describe the whole launch."""
+    patterns = list(patterns)
+    n = len(patterns)
+    o, _ = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, False,
+                      stream, cache_bytes, package_bytes, policy, prefetch, order)
+    hs = (C.c_void_p * max(n, 1))(*[q._h.value for q in patterns])
+    rs = (Result * max(n, 1))()
+    _check(lib().pm_match_batch(g._h, hs, n, C.byref(o), rs), "match_batch")
+    return [_result(rs[i]) for i in range(n)]
 
 
 def pattern_text(k: int, edges, labels=None) -> str:

@@ -131,6 +131,7 @@ int pm_graph_generate_chunglu(uint64_t n, double gamma, double dmax, double avg,
 int pm_graph_set_uniform_labels(pm_graph* g, uint32_t nlabels, uint64_t seed) {
   if (!g || nlabels == 0 || nlabels > 32) return PM_ERR_ARG;
   assign_uniform_labels(&g->h, nlabels, seed);
+  g->stats_ok = false;
   return device_update_labels(g);
 }
 
@@ -240,11 +241,16 @@ int pm_match(pm_graph* g, const pm_pattern* p, const pm_opts* opts, pm_result* r
 int pm_match_batch(pm_graph* g, const pm_pattern* const* p, int32_t n, const pm_opts* opts,
                    pm_result* results) {
   if (!g || !p || n < 0 || (n && !results)) return PM_ERR_ARG;
+  if (n == 0) return PM_OK;
+  pm_opts def;
+  pm_opts_default(&def);
+  if (!opts) opts = &def;
+  std::vector<const Plan*> plans((size_t)n);
   for (int32_t i = 0; i < n; i++) {
-    int rc = pm_match(g, p[i], opts, &results[i]);
-    if (rc) return rc;
+    if (!p[i]) return PM_ERR_ARG;
+    plans[i] = &p[i]->plan;
   }
-  return PM_OK;
+  return device_match_batch(g, plans.data(), n, opts, results);
 }
 
 }  // extern "C"

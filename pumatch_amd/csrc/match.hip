@@ -48,11 +48,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "paging.hpp"
@@ -129,6 +132,8 @@ struct Task {
   unsigned long long edge;
   uint32_t lo, hi;
   uint32_t r, c;  // the job's endpoints
+  uint32_t pat;   // pattern of a batched launch
+  uint32_t pad;
 };
 
 // A deferred piece of search.  depth 0: the whole job (edge, r, c) over its
@@ -144,7 +149,7 @@ struct Resume {
   uint32_t idx[kMaxDepth];
   uint32_t phi[kMaxDepth];
   uint32_t blk;
-  uint32_t pad;
+  uint32_t pat;  // pattern of a batched launch
 };
 
 struct DeferOut {
@@ -176,6 +181,8 @@ struct WaveState {
   uint32_t hash[kHashCap];
   unsigned long long je;  // the job being searched: edge and vertex-2 range
   uint32_t jlo, jhi;
+  uint32_t pat;           // its pattern (batched launches)
+  unsigned long long acc_count, acc_jobs;  // phase 1: totals of pattern `pat`
   uint32_t blk;     // package of the last row found missing
   int32_t rdepth;   // resume input: depth, candidate indices, range end
   uint32_t ridx[kMaxDepth];
@@ -562,6 +569,7 @@ __device__ void emit_one(const DeferOut& O, const WaveState& ws, int depth, uint
     if (depth >= 1 && depth < kMaxDepth) R.idx[depth] = a;
     for (int d = 0; d <= depth && d < kMaxDepth; d++) R.phi[d] = ws.phi[d];
     R.blk = blk;
+    R.pat = ws.pat;
   }
 }
 
@@ -588,6 +596,7 @@ __device__ void emit_lanes(const DeferOut& O, const WaveState& ws, int depth, bo
   if (depth < kMaxDepth) R.idx[depth] = i;
   for (int d = 0; d <= depth && d < kMaxDepth; d++) R.phi[d] = ws.phi[d];
   R.blk = blk;
+  R.pat = ws.pat;
 }
 
 // The last two core vertices u1 = d+1, u2 = d+2 (d = k-3), fused: weighted
@@ -960,6 +969,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
           tk.hi = rank + 1 == nstart ? 0xFFFFFFFFu : 0u;
           tk.r = r;
           tk.c = c;
+          tk.pat = ws.pat;
           tasks[tb + rank] = tk;
         }
         if (pass == 1 && sm && rank0 == 0 && keep_end == en)
@@ -984,6 +994,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
             tk.hi = 1;
             tk.r = r;
             tk.c = c;
+            tk.pat = ws.pat;
             tasks[t] = tk;
           }
           break;
@@ -1089,22 +1100,30 @@ __device__ __forceinline__ uint32_t source_full(const KGraph& g, uint64_t e) {
   return source_of(g, e, g.src64[e >> 6]);
 }
 
+// One launch runs npat patterns (a batch; npat = 1 for a single pattern).
+// Phase-1 work units are (pattern, chunk of edges) pairs numbered pattern-
+// major and claimed from one cursor; tasks and deferred pieces carry their
+// pattern.
 struct Launch {
   KGraph g;
-  const KPlan* plan;
-  unsigned long long* cursor;  // [0] edges [1] task claim [2] tasks [3] resume claim [4] deferred
-  uint64_t edge_lo, edge_hi;
+  const KPlan* plans;
+  uint32_t npat;
   uint32_t chunk;
+  uint64_t nchunk;             // edge chunks per pattern
+  unsigned long long* cursor;  // [0] units [1] task claim [2] tasks [3] piece claim [4] pieces
+  uint64_t edge_lo, edge_hi;
   uint32_t cap;
+  uint32_t pad;
   uint32_t* arena;
   uint64_t arena_stride;
-  unsigned long long* out;       // count, jobs, alg/ops edges, alg/ops tasks, alg fold
-  unsigned long long* per_edge;  // optional, indexed e - edge_lo
+  unsigned long long* out;       // statistics: alg/ops per phase
+  unsigned long long* pcount;    // per pattern: count, jobs
+  unsigned long long* per_edge;  // optional (npat == 1), indexed e - edge_lo
   Task* tasks;
   uint64_t task_cap;
-  const Resume* rin;  // resume descriptors of this round
+  const Resume* rin;  // deferred pieces resumed this round
   uint64_t nrin;
-  Resume* rout;  // deferred searches for the next round
+  Resume* rout;  // deferred pieces for the next round
   uint64_t rout_cap;
 };
 
@@ -1121,22 +1140,38 @@ template <bool PG>
 __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
-  const KPlan& P = *L.plan;
   const uint32_t lane = threadIdx.x;
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
-  unsigned long long total = 0, jobs = 0;
   Counters ct;
+  // per-pattern totals accumulate in LDS (lane 0) and are flushed when the
+  // wave moves on to another pattern's chunks
+  if (lane == 0) {
+    ws.pat = 0xFFFFFFFFu;
+    ws.acc_count = ws.acc_jobs = 0;
+  }
+  __syncthreads();
   const DeferOut O = defer_out(L);
-  const uint64_t span = L.edge_hi - L.edge_lo;
+  const uint64_t units = (uint64_t)L.npat * L.nchunk;
   for (;;) {
     // out of core: stop taking new jobs once the piece buffer is half full;
-    // a later round continues the edge scan from the cursor
+    // a later round continues the scan from the cursor
     if (PG && uni64(*(volatile unsigned long long*)O.count) > O.soft) break;
-    unsigned long long c0 = 0;
-    if (lane == 0) c0 = atomicAdd(&L.cursor[0], (unsigned long long)L.chunk);
-    c0 = uni64(__shfl(c0, 0));
-    if (c0 >= span) break;
-    const uint64_t e0 = L.edge_lo + c0;
+    unsigned long long u = 0;
+    if (lane == 0) u = atomicAdd(&L.cursor[0], 1ull);
+    u = uni64(__shfl(u, 0));
+    if (u >= units) break;
+    const uint32_t p = uni((uint32_t)(u / L.nchunk));
+    if (lane == 0 && p != ws.pat) {
+      if (ws.pat != 0xFFFFFFFFu && (ws.acc_count | ws.acc_jobs)) {
+        atomicAdd(&L.pcount[2 * ws.pat], ws.acc_count);
+        atomicAdd(&L.pcount[2 * ws.pat + 1], ws.acc_jobs);
+      }
+      ws.acc_count = ws.acc_jobs = 0;
+      ws.pat = p;
+    }
+    __syncthreads();
+    const KPlan& P = L.plans[p];
+    const uint64_t e0 = L.edge_lo + (u % L.nchunk) * L.chunk;
     const uint64_t e1 = min(e0 + L.chunk, L.edge_hi);
     uint32_t rlo = uni(source_full(g, e0));
     for (uint64_t eb = e0; eb < e1; eb += kWave) {
@@ -1161,34 +1196,36 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
       while (jm) {
         const int i = __builtin_ctzll(jm);
         jm &= jm - 1;
-        jobs++;
         const uint32_t jr = uni(ws.jr[i]), jc = uni(ws.jc[i]);
         const unsigned long long cnt =
             run_job<PG>(g, P, ws, arena, L.cap, lane, jr, jc, eb + i, 0, kInf, false,
                         L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
-        total += cnt;
-        if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[eb + i - L.edge_lo], cnt);
+        if (lane == 0) {
+          ws.acc_count += cnt;
+          ws.acc_jobs++;
+          if (L.per_edge && cnt) atomicAdd(&L.per_edge[eb + i - L.edge_lo], cnt);
+        }
       }
       __syncthreads();
     }
   }
   if (lane == 0) {
-    atomicAdd(&L.out[0], total);
-    atomicAdd(&L.out[1], jobs);
+    if (ws.pat != 0xFFFFFFFFu && (ws.acc_count | ws.acc_jobs)) {
+      atomicAdd(&L.pcount[2 * ws.pat], ws.acc_count);
+      atomicAdd(&L.pcount[2 * ws.pat + 1], ws.acc_jobs);
+    }
     atomicAdd(&L.out[2], (unsigned long long)ct.alg);
     atomicAdd(&L.out[3], (unsigned long long)ct.nops);
   }
 }
 
-// Range tasks (phase 2), then -- in resume rounds -- resume descriptors.
+// Range tasks (phase 2).
 template <bool PG>
 __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_tasks(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
-  const KPlan& P = *L.plan;
   const uint32_t lane = threadIdx.x;
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
-  unsigned long long total = 0;
   Counters ct;
   const DeferOut O = defer_out(L);
   const uint64_t ntask = L.tasks ? min((uint64_t)L.cursor[2], L.task_cap) : 0;
@@ -1205,27 +1242,27 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_tasks(Launch L) {
     if (thi == 0) thi = uni(L.tasks[t + 1].lo);  // same job: slots are contiguous
     const uint32_t r = uni(L.tasks[t].r);
     const uint32_t c = uni(L.tasks[t].c);
-    const unsigned long long cnt = run_job<PG>(g, P, ws, arena, L.cap, lane, r, c, e, tlo, thi,
-                                               false, false, nullptr, nullptr, 0, O, ct);
-    total += cnt;
+    const uint32_t p = uni(L.tasks[t].pat);
+    ws.pat = p;
+    const unsigned long long cnt =
+        run_job<PG>(g, L.plans[p], ws, arena, L.cap, lane, r, c, e, tlo, thi, false, false,
+                    nullptr, nullptr, 0, O, ct);
+    if (lane == 0 && cnt) atomicAdd(&L.pcount[2 * p], cnt);
     if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
   }
   if (lane == 0) {
-    atomicAdd(&L.out[0], total);
     atomicAdd(&L.out[4], (unsigned long long)ct.alg);
     atomicAdd(&L.out[5], (unsigned long long)ct.nops);
   }
 }
 
-// Resume round: every descriptor of the previous round, may split again.
+// Resume round: the admitted deferred pieces (a depth-0 piece may split).
 template <bool PG>
 __global__ __launch_bounds__(64, PG ? 4 : 6) void k_resume(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
-  const KPlan& P = *L.plan;
   const uint32_t lane = threadIdx.x;
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
-  unsigned long long total = 0;
   Counters ct;
   const DeferOut O = defer_out(L);
   for (;;) {
@@ -1237,18 +1274,19 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_resume(Launch L) {
     const uint32_t r = uni(L.rin[t].r), c = uni(L.rin[t].c);
     const uint32_t rlo = uni(L.rin[t].lo), rhi = uni(L.rin[t].hi);
     const int depth = (int)uni((uint32_t)L.rin[t].depth);
+    const uint32_t p = uni(L.rin[t].pat);
+    ws.pat = p;
     ws.rdepth = depth;
     for (int d = 1; d <= depth && d < kMaxDepth; d++) ws.ridx[d] = uni(L.rin[t].idx[d]);
     for (int d = 0; d <= depth && d < kMaxDepth; d++) ws.phi[d] = uni(L.rin[t].phi[d]);
     ws.rend = uni(L.rin[t].end);
     const unsigned long long cnt =
-        run_job<PG>(g, P, ws, arena, L.cap, lane, r, c, e, rlo, rhi, true,
+        run_job<PG>(g, L.plans[p], ws, arena, L.cap, lane, r, c, e, rlo, rhi, true,
                     depth == 0 && L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
-    total += cnt;
+    if (lane == 0 && cnt) atomicAdd(&L.pcount[2 * p], cnt);
     if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
   }
   if (lane == 0) {
-    atomicAdd(&L.out[0], total);
     atomicAdd(&L.out[2], (unsigned long long)ct.alg);
     atomicAdd(&L.out[3], (unsigned long long)ct.nops);
   }
@@ -1351,7 +1389,10 @@ struct Workspace {
   size_t arena_bytes = 0;
   unsigned long long* cursor = nullptr;  // 8 words
   unsigned long long* out = nullptr;     // 8 words
-  KPlan* plan = nullptr;
+  void* plans = nullptr;                 // KPlan per pattern of the launch
+  size_t plans_bytes = 0;
+  void* pcount = nullptr;  // per pattern: count, jobs
+  size_t pcount_bytes = 0;
   Task* tasks = nullptr;
   uint64_t task_cap = 0;
   void* tables = nullptr;  // fold / weight tables
@@ -1372,7 +1413,6 @@ int ensure_workspace(int device) {
   HIP_OK(hipStreamCreateWithFlags(&g_ws.copy_stream, hipStreamNonBlocking));
   HIP_OK(hipMalloc(&g_ws.cursor, 8 * sizeof(unsigned long long)));
   HIP_OK(hipMalloc(&g_ws.out, 8 * sizeof(unsigned long long)));
-  HIP_OK(hipMalloc(&g_ws.plan, sizeof(KPlan)));
   HIP_OK(hipEventCreate(&g_ws.ev0));
   HIP_OK(hipEventCreate(&g_ws.ev1));
   HIP_OK(hipEventCreate(&g_ws.evf));
@@ -1729,6 +1769,26 @@ int load_packages(PagedGraph* pg, const std::vector<uint32_t>& want, std::vector
 }
 
 int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res) {
+  const Plan* one = &plan;
+  return device_match_batch(g, &one, 1, opts, res);
+}
+
+// Identity of a pendant-tree table: F_u depends only on u's label and the
+// tables of its children, so equal subtrees of different patterns (or of one
+// pattern) share one table.
+static std::string fold_signature(const Plan& p, int u, bool product) {
+  std::vector<std::string> kids;
+  for (int c = 0; c < p.k; c++)
+    if (p.fold_children[u] >> c & 1u) kids.push_back(fold_signature(p, c, false));
+  std::sort(kids.begin(), kids.end());
+  std::string s = product ? "P(" : ("L" + std::to_string(p.vlabel[u]) + "(");
+  for (const std::string& k : kids) s += k + ",";
+  return s + ")";
+}
+
+int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm_opts* opts,
+                       pm_result* res) {
+  if (npat <= 0) return PM_OK;
   const bool paged = opts->cache_bytes > 0;
   int rc = device_upload_rows(g, !paged);
   if (rc) return rc;
@@ -1742,13 +1802,42 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   rc = ensure_workspace(g->device);
   if (rc) return rc;
   const HostGraph& h = g->h;
-  if (plan.labeled && h.labels.empty()) {
-    set_error("labeled matching needs vertex labels (open the graph with PM_GRAPH_LABELS)");
+  for (int q = 0; q < npat; q++)
+    if (plans[q]->labeled && h.labels.empty()) {
+      set_error("labeled matching needs vertex labels (open the graph with PM_GRAPH_LABELS)");
+      return PM_ERR_ARG;
+    }
+  if (opts->per_edge && npat != 1) {
+    set_error("per-edge counts are available for a single pattern only");
     return PM_ERR_ARG;
   }
-  KPlan kp;
+  // labeled plans run in the cheapest core order (the count does not
+  // depend on it); `plans` then points at the reordered copies
+  if (!g->stats_ok) {
+    g->stats = order_stats(h);
+    g->stats_ok = true;
+  }
+  std::vector<Plan> reordered;
+  std::vector<const Plan*> use(plans, plans + npat);
+  if (opts->order == 0) {
+    reordered.reserve(npat);
+    for (int q = 0; q < npat; q++)
+      if (plans[q]->labeled && plans[q]->kc >= 4) {
+        reordered.push_back(*plans[q]);
+        reorder_core(&reordered.back(), g->stats);
+        use[q] = &reordered.back();
+      }
+  }
+  plans = use.data();
+  std::vector<KPlan> kps(npat);
   int nreg = 0;
-  build_kplan(plan, !paged && g->d.nl > 0, &kp, &nreg);
+  bool any_split = false;
+  for (int q = 0; q < npat; q++) {
+    int nr = 0;
+    build_kplan(*plans[q], !paged && g->d.nl > 0, &kps[q], &nr);
+    nreg = std::max(nreg, nr);
+    any_split |= plans[q]->kc >= 4;
+  }
   hipStream_t st = opts->stream ? (hipStream_t)opts->stream : g_ws.stream;
   // default: every wave the chip can hold at once (persistent grid)
   if (g_ws.resident == 0) {
@@ -1763,27 +1852,60 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
     return PM_ERR_ARG;
   }
   const uint32_t chunk =
-      opts->chunk > 0 ? (uint32_t)opts->chunk
-                      : (uint32_t)std::max<uint64_t>(
-                            1, std::min<uint64_t>(64, (hi - lo) / ((uint64_t)waves * 8)));
-  kp.split = opts->split > 0 ? (uint32_t)opts->split : 8192u;
-  kp.hub_factor = opts->hub_factor > 0 ? (uint32_t)opts->hub_factor : 8u;
+      opts->chunk > 0
+          ? (uint32_t)opts->chunk
+          : (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(64, (hi - lo) * npat / ((uint64_t)waves * 8)));
+  const uint64_t nchunk = (hi - lo + chunk - 1) / chunk;
+  for (KPlan& kp : kps) {
+    kp.split = opts->split > 0 ? (uint32_t)opts->split : 8192u;
+    kp.hub_factor = opts->hub_factor > 0 ? (uint32_t)opts->hub_factor : 8u;
+  }
 
-  // fold / weight tables: one u64[n] per folded vertex and per weighted
-  // core vertex
+  // fold / weight tables (u64[n] each): one per distinct pendant subtree and
+  // per distinct weighted core vertex, over all patterns of the launch
+  struct TableJob {
+    int plan, u;
+    bool product;
+    std::vector<int> child;  // table ids
+  };
   const uint64_t nn = std::max<uint64_t>(h.n, 1);
-  int table_of[kMaxK];
-  int ntab = 0;
-  for (int u = 0; u < kMaxK; u++) table_of[u] = -1;
-  for (int i = 0; i < plan.nfold; i++) table_of[plan.fold_v[i]] = ntab++;
-  int core_tab[kMaxK];
-  for (int i = 0; i < plan.kc; i++) core_tab[i] = plan.fold_children[plan.core_v[i]] ? ntab++ : -1;
+  std::map<std::string, int> table_id;
+  std::vector<TableJob> table_jobs;
+  std::vector<std::array<int, kMaxK>> wt_tab(npat);
+  for (int q = 0; q < npat; q++) {
+    const Plan& pl = *plans[q];
+    int table_of[kMaxK];
+    for (int u = 0; u < kMaxK; u++) table_of[u] = -1;
+    auto make = [&](int u, bool product) {
+      const std::string sig = fold_signature(pl, u, product);
+      auto it = table_id.find(sig);
+      if (it != table_id.end()) return it->second;
+      TableJob j;
+      j.plan = q;
+      j.u = u;
+      j.product = product;
+      for (int c = 0; c < pl.k; c++)
+        if (pl.fold_children[u] >> c & 1u) j.child.push_back(table_of[c]);
+      const int id = (int)table_jobs.size();
+      table_jobs.push_back(j);
+      table_id.emplace(sig, id);
+      return id;
+    };
+    for (int i = 0; i < pl.nfold; i++) table_of[pl.fold_v[i]] = make(pl.fold_v[i], false);
+    wt_tab[q].fill(-1);
+    for (int i = 0; i < pl.kc; i++)
+      if (pl.fold_children[pl.core_v[i]]) wt_tab[q][i] = make(pl.core_v[i], true);
+  }
+  const int ntab = (int)table_jobs.size();
   if (ntab) {
     rc = grow(&g_ws.tables, &g_ws.table_bytes, sizeof(unsigned long long) * ntab * nn);
     if (rc) return rc;
   }
   auto table = [&](int t) { return (unsigned long long*)g_ws.tables + (size_t)t * nn; };
-  for (int i = 0; i < plan.kc; i++) kp.wt[i] = core_tab[i] >= 0 ? table(core_tab[i]) : nullptr;
+  for (int q = 0; q < npat; q++)
+    for (int i = 0; i < plans[q]->kc; i++)
+      kps[q].wt[i] = wt_tab[q][i] >= 0 ? table(wt_tab[q][i]) : nullptr;
 
   const uint32_t cap = std::max<uint32_t>(h.max_degree, 1);
   const uint64_t stride = (uint64_t)std::max(nreg, 1) * cap;
@@ -1794,7 +1916,11 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
     rc = grow(&g_ws.resume[i], &g_ws.resume_bytes[i], sizeof(Resume) * rcap);
     if (rc) return rc;
   }
-  HIP_OK(hipMemcpyAsync(g_ws.plan, &kp, sizeof(kp), hipMemcpyHostToDevice, st));
+  rc = grow(&g_ws.plans, &g_ws.plans_bytes, sizeof(KPlan) * npat);
+  if (rc) return rc;
+  rc = grow(&g_ws.pcount, &g_ws.pcount_bytes, sizeof(unsigned long long) * 2 * npat);
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(g_ws.plans, kps.data(), sizeof(KPlan) * npat, hipMemcpyHostToDevice, st));
   unsigned long long* d_per_edge = nullptr;
   if (opts->per_edge && hi > lo)
     HIP_OK(hipMallocAsync((void**)&d_per_edge, sizeof(unsigned long long) * (hi - lo), st));
@@ -1827,7 +1953,10 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   Launch L;
   std::memset(&L, 0, sizeof(L));
   L.g = kg;
-  L.plan = g_ws.plan;
+  L.plans = (const KPlan*)g_ws.plans;
+  L.npat = (uint32_t)npat;
+  L.nchunk = nchunk;
+  L.pcount = (unsigned long long*)g_ws.pcount;
   L.cursor = g_ws.cursor;
   L.edge_lo = lo;
   L.edge_hi = hi;
@@ -1836,7 +1965,7 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   L.arena = (uint32_t*)g_ws.arena;
   L.arena_stride = stride;
   L.out = g_ws.out;
-  L.tasks = (plan.kc >= 4 && opts->split >= 0) ? g_ws.tasks : nullptr;
+  L.tasks = (any_split && opts->split >= 0) ? g_ws.tasks : nullptr;
   L.task_cap = g_ws.task_cap;
   L.rout_cap = rcap;
   const int reps = opts->repeats > 0 ? opts->repeats : 1;
@@ -1849,6 +1978,7 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
   for (int r = 0; r < reps; r++) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor, 8, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.out, 8, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, L.pcount, 2ull * npat, 0ull);
     L.per_edge = (r == 0) ? d_per_edge : nullptr;
     if (L.per_edge)
       hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, st, d_per_edge, hi - lo, 0ull);
@@ -1884,32 +2014,24 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
                          (pg->nslots + 3) / 4, 0u);
     }
     HIP_OK(hipEventRecord(g_ws.ev0, st));
-    // pendant-tree tables, children first
-    for (int i = 0; i < plan.nfold; i++) {
-      const int u = plan.fold_v[i];
+    // pendant-tree tables, children first (table ids are in that order)
+    for (const TableJob& j : table_jobs) {
+      const Plan& pl = *plans[j.plan];
+      const int id = (int)(&j - table_jobs.data());
       FoldArgs A;
       std::memset(&A, 0, sizeof(A));
       A.g = kg;
-      A.want = (uint32_t)plan.vlabel[u];
-      for (int c = 0; c < plan.k; c++)
-        if (plan.fold_children[u] >> c & 1u) A.child[A.nchild++] = table(table_of[c]);
-      A.out = table(table_of[u]);
+      A.want = (uint32_t)pl.vlabel[j.u];
+      for (int c : j.child) A.child[A.nchild++] = table(c);
+      A.out = table(id);
       A.bytes = g_ws.out + 6;
-      if (h.n) {
+      if (!h.n) continue;
+      if (j.product) {
+        hipLaunchKernelGGL(k_product, dim3(1024), dim3(256), 0, st, h.n, A);
+      } else {
         hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, st, A.out, h.n, 0ull);
         if (h.m) hipLaunchKernelGGL(k_fold, dim3(fold_blocks), dim3(kWave), 0, st, A);
       }
-    }
-    for (int i = 0; i < plan.kc; i++) {
-      if (core_tab[i] < 0) continue;
-      FoldArgs A;
-      std::memset(&A, 0, sizeof(A));
-      const int u = plan.core_v[i];
-      for (int c = 0; c < plan.k; c++)
-        if (plan.fold_children[u] >> c & 1u) A.child[A.nchild++] = table(table_of[c]);
-      A.out = table(core_tab[i]);
-      A.bytes = g_ws.out + 6;
-      if (h.n) hipLaunchKernelGGL(k_product, dim3(1024), dim3(256), 0, st, h.n, A);
     }
     HIP_OK(hipEventRecord(g_ws.evf, st));
     L.rout = (Resume*)g_ws.resume[1];
@@ -1935,7 +2057,7 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
       // only those.  Every admitted search then passes its stopping point,
       // so the rounds terminate; nothing is dropped, so counts are exact.
       const PackageLayout& PL = pg->layout;
-      const uint64_t span = hi - lo;
+      const uint64_t span = (uint64_t)npat * nchunk;  // phase-1 units
       const size_t admit_cap = 1u << 18;  // pieces resumed per round
       std::vector<Resume> pending, fresh;
       for (;;) {
@@ -2062,8 +2184,11 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
     min_ms = std::min(min_ms, t);
   }
   unsigned long long out[8], cur[8];
+  std::vector<unsigned long long> pc(2 * (size_t)npat);
   HIP_OK(hipMemcpyAsync(out, g_ws.out, sizeof(out), hipMemcpyDeviceToHost, st));
   HIP_OK(hipMemcpyAsync(cur, g_ws.cursor, sizeof(cur), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(pc.data(), g_ws.pcount, sizeof(unsigned long long) * pc.size(),
+                        hipMemcpyDeviceToHost, st));
   if (d_per_edge) {
     HIP_OK(hipMemcpyAsync(opts->per_edge + lo, d_per_edge, sizeof(unsigned long long) * (hi - lo),
                           hipMemcpyDeviceToHost, st));
@@ -2074,28 +2199,32 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
     set_error("internal: a resident-row run deferred work");
     return PM_ERR_DEVICE;
   }
-  uint64_t total = out[0];
-  if (!plan.labeled) total *= (uint64_t)plan.aut;  // test.cu:198
-  std::memset(res, 0, sizeof(*res));
-  res->kernel_ms = sum_ms / reps;
-  res->min_ms = min_ms;
-  res->count = total;
-  res->jobs = out[1];
-  res->alg_bytes = out[2] + out[4] + out[6];
-  res->set_ops = out[3] + out[5];
-  for (int i = 0; i < 3; i++) res->phase_ms[i] = phase[i];
-  res->phase_alg_bytes[0] = out[6];
-  res->phase_alg_bytes[1] = out[2];
-  res->phase_alg_bytes[2] = out[4];
-  res->tasks = std::min<uint64_t>(cur[2], g_ws.task_cap);
-  res->waves = waves;
-  res->chunk = (int32_t)chunk;
-  res->link_bytes = link_bytes;
-  res->link_ms = link_ms;
-  res->misses = misses;
-  res->deferred = deferred;
-  res->rounds = rounds;
-  res->packages = (int32_t)np;
+  for (int q = 0; q < npat; q++) {
+    pm_result* rq = &res[q];
+    std::memset(rq, 0, sizeof(*rq));
+    uint64_t total = pc[2 * q];
+    if (!plans[q]->labeled) total *= (uint64_t)plans[q]->aut;  // test.cu:198
+    rq->count = total;
+    rq->jobs = pc[2 * q + 1];
+    // launch-wide figures (the whole batch)
+    rq->kernel_ms = sum_ms / reps;
+    rq->min_ms = min_ms;
+    rq->alg_bytes = out[2] + out[4] + out[6];
+    rq->set_ops = out[3] + out[5];
+    for (int i = 0; i < 3; i++) rq->phase_ms[i] = phase[i];
+    rq->phase_alg_bytes[0] = out[6];
+    rq->phase_alg_bytes[1] = out[2];
+    rq->phase_alg_bytes[2] = out[4];
+    rq->tasks = std::min<uint64_t>(cur[2], g_ws.task_cap);
+    rq->waves = waves;
+    rq->chunk = (int32_t)chunk;
+    rq->link_bytes = link_bytes;
+    rq->link_ms = link_ms;
+    rq->misses = misses;
+    rq->deferred = deferred;
+    rq->rounds = rounds;
+    rq->packages = (int32_t)np;
+  }
   return PM_OK;
 }
 

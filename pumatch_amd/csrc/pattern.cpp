@@ -11,6 +11,7 @@
 // intersections with copy slots (src/pattern.h:143-196, convert2oned
 // :329-355); the counts do not depend on how the intersections are shared.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <sstream>
 
@@ -195,6 +196,12 @@ int build_plan(Plan* pp, std::string* err) {
   p.kc = 0;
   for (int u = 0; u < k; u++)
     if (alive[u]) p.core_v[p.kc++] = u;
+  return build_core(&p, err);
+}
+
+// Core arrays and slot operations for the core order in p.core_v.
+int build_core(Plan* pp, std::string* err) {
+  Plan& p = *pp;
   for (int i = 0; i < p.kc; i++) {
     const int u = p.core_v[i];
     p.clabel[i] = p.vlabel[u];
@@ -258,6 +265,82 @@ int build_plan(Plan* pp, std::string* err) {
     }
   }
   return PM_OK;
+}
+
+// Labeled plans: choose the order of core vertices 2..kc-1.  A labeled
+// count is the number of homomorphisms with phi(0) = r, phi(1) = c summed
+// over the jobs, so it does not depend on the order in which vertices 2..
+// are matched (the job filter keeps the file-order degrees, see deg[]);
+// only the work does.  Among all orders in which every vertex has an
+// earlier neighbour, take the one with the fewest estimated partial matches
+// at the enumerated depths 2..kc-2 (the last vertex is counted, not
+// enumerated).  A vertex with b earlier neighbours and label l has about
+// dnb * f(l) * q^(b-1) candidates, dnb the mean neighbour degree, f(l) the
+// label's share and q the chance that a candidate also lies in one more
+// neighbour list.  Ties keep file order.
+void reorder_core(Plan* pp, const OrderStats& st) {
+  Plan& p = *pp;
+  if (!p.labeled || p.kc < 4) return;
+  const int kc = p.kc;
+  const double n = std::max(st.n, 1.0);
+  const double dnb = std::max(st.dnb, 1.0);
+  const double q = std::min(0.5, std::max(0.02, 4.0 * dnb / n));
+  auto share = [&](int l) {
+    return (l >= 0 && l < (int)st.label_share.size()) ? std::max(st.label_share[l], 1e-6) : 1e-6;
+  };
+  std::vector<int> rest(p.core_v + 2, p.core_v + kc), best = rest;
+  auto cost_of = [&](const std::vector<int>& ord, double* out) {
+    std::vector<int> seq = {p.core_v[0], p.core_v[1]};
+    seq.insert(seq.end(), ord.begin(), ord.end());
+    double prefixes = 1.0, cost = 0.0;
+    for (int i = 2; i < kc; i++) {
+      const int u = seq[i];
+      int b = 0;
+      for (int j = 0; j < i; j++) b += p.adj[u][seq[j]] != 0;
+      if (b == 0) return false;
+      if (i <= kc - 2) {
+        prefixes *= dnb * share(p.vlabel[u]) * std::pow(q, b - 1);
+        cost += prefixes;
+      }
+    }
+    *out = cost;
+    return true;
+  };
+  double best_cost = 0;
+  if (!cost_of(best, &best_cost)) return;  // file order is always valid
+  std::sort(rest.begin(), rest.end());
+  do {
+    double c;
+    if (cost_of(rest, &c) && c < best_cost * 0.999) {
+      best_cost = c;
+      best = rest;
+    }
+  } while (std::next_permutation(rest.begin(), rest.end()));
+  const Plan file_order = p;
+  for (int i = 2; i < kc; i++) p.core_v[i] = best[i - 2];
+  std::string err;
+  if (build_core(&p, &err) != PM_OK) p = file_order;
+}
+
+OrderStats order_stats(const HostGraph& h) {
+  OrderStats st;
+  st.n = (double)h.n;
+  st.m = (double)h.m;
+  double s2 = 0;
+  for (uint64_t v = 0; v < h.n; v++) {
+    const double d = (double)(h.rowptr[v + 1] - h.rowptr[v]);
+    s2 += d * d;
+  }
+  st.dnb = h.m ? s2 / (double)h.m : 0.0;
+  if (!h.labels.empty()) {
+    uint32_t mx = 0;
+    for (uint32_t l : h.labels) mx = std::max(mx, l);
+    st.label_share.assign((size_t)std::min<uint32_t>(mx, 1u << 20) + 1, 0.0);
+    for (uint32_t l : h.labels)
+      if (l < st.label_share.size()) st.label_share[l] += 1.0;
+    for (double& x : st.label_share) x /= std::max(st.n, 1.0);
+  }
+  return st;
 }
 
 }  // namespace pm

@@ -44,11 +44,22 @@ struct DeviceGraph {
 
 }  // namespace pm
 
+namespace pm {
+// Graph figures the planner's cost model reads.
+struct OrderStats {
+  double n = 0, m = 0;
+  double dnb = 0;                   // mean degree of a neighbour: sum d^2 / sum d
+  std::vector<double> label_share;  // fraction of vertices per label
+};
+}  // namespace pm
+
 struct pm_graph {
   pm::HostGraph h;
   pm::DeviceGraph d;
   bool on_device = false;
   int device = 0;
+  bool stats_ok = false;  // cleared when labels change
+  pm::OrderStats stats;
 };
 
 namespace pm {
@@ -116,6 +127,10 @@ namespace pm {
 
 int parse_pattern_text(const std::string& text, int labeled, Plan* out, std::string* err);
 int build_plan(Plan* p, std::string* err);
+int build_core(Plan* p, std::string* err);
+// labeled plans: cost-based order of core vertices 2..kc-1 (same count)
+void reorder_core(Plan* p, const OrderStats& st);
+OrderStats order_stats(const HostGraph& h);
 
 // generators (gen.cpp)
 int generate_rmat(HostGraph* g, int scale, double edge_factor, double a, double b, double c,
@@ -135,5 +150,9 @@ void build_label_grouped(const HostGraph& g, std::vector<uint32_t>* lcol,
 int device_upload(pm_graph* g);
 void device_release(pm_graph* g);
 int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res);
+// One launch over several patterns; res[i] gets pattern i's count and jobs
+// and the launch-wide timings.
+int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm_opts* opts,
+                       pm_result* res);
 
 }  // namespace pm

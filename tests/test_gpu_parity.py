@@ -103,7 +103,8 @@ def test_schedule_invariance():
         p = pm.Pattern.parse(text, labeled)
         base = pm.match(g, p, per_edge=True)
         for kw in (dict(waves=64), dict(waves=3000, chunk=1), dict(chunk=7, split=-1),
-                   dict(split=1), dict(split=5, hub_factor=1), dict(hub_factor=1000)):
+                   dict(split=1), dict(split=5, hub_factor=1), dict(hub_factor=1000),
+                   dict(order="file")):
             print(f"[invariance] labeled={labeled} {kw}", flush=True)
             r = pm.match(g, p, per_edge=True, **kw)
             assert r.count == base.count
@@ -191,3 +192,16 @@ def test_vertex_relabelling_invariance():
     for edges in (HT_EDGES, K5P_EDGES):
         p = pm.Pattern.parse(g_text(7, edges_to_adj(7, edges)), False)
         assert pm.match(h1, p).count == pm.match(h2, p).count
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_core_order_keeps_per_edge_counts(rmat9, seed):
+    """Labeled plans may match vertices 2.. in any order; per-job counts are
+    the same in file order and in the planner's order, and equal the
+    oracle's (file order)."""
+    from pumatch_amd.workloads import batch_patterns
+    text = batch_patterns(6, seed=40 + seed, nlabels=2, kmin=6, kmax=7)[seed]
+    p = pm.Pattern.parse(text, True)
+    a = pm.match(rmat9, p, per_edge=True, order="file")
+    b = check(rmat9, text, True)
+    assert a.count == b.count and np.array_equal(a.per_edge, b.per_edge)
