@@ -71,6 +71,9 @@ typedef struct {
   int32_t order;        /* labeled patterns: 0 = cost-based order of pattern
                            vertices 2.. (same count, less work), 1 = file
                            order as the reference (src/pattern.h:67-100) */
+  int32_t sample;       /* > 1: jobs only from every sample-th block of 4096
+                           CSR edges (timing samples; the count is then that
+                           of the sampled jobs) */
 } pm_opts;
 
 typedef struct {
@@ -185,6 +188,35 @@ int pm_match(pm_graph* g, const pm_pattern* p, const pm_opts* opts, pm_result* r
  * pattern file (plot/plot_fig11.sh:112-123).  opts->per_edge needs n == 1. */
 int pm_match_batch(pm_graph* g, const pm_pattern* const* p, int32_t n, const pm_opts* opts,
                    pm_result* results);
+
+/* -- autotuning ------------------------------------------------------------ */
+
+typedef struct {
+  int32_t sample;      /* job sample for the timed trials (0 = about 1M edges) */
+  int32_t max_orders;  /* labeled patterns: core orders timed per pattern (0 = 4) */
+  int32_t repeats;     /* launches per trial, fastest counts (0 = 2) */
+} pm_tune_opts;
+
+typedef struct {
+  double base_ms;   /* sampled batch time with the caller's options */
+  double best_ms;   /* sampled batch time with the chosen options */
+  int32_t trials;   /* timed launches */
+  int32_t reordered; /* patterns whose measured-fastest order is not the file order */
+  int32_t sample;   /* sample used */
+  int32_t pad;
+} pm_tune_report;
+
+/* Pick, by timing sampled launches on g, (1) for each labeled pattern the
+ * order of its core vertices 2.. (file order and the planner's cheapest
+ * candidates; remembered in the pattern handle for this graph) and (2) the
+ * launch granularities of the whole batch: chunk (jobs per claim, the steal
+ * granularity), split (range-task size) and hub_factor (fused-tail strategy
+ * switch).  *best = *base with those fields set.  Counts never depend on any
+ * of them; every trial's counts are checked against the first and a
+ * mismatch fails with PM_ERR_DEVICE.  No counterpart in the reference, whose
+ * granularities are compile-time constants (src/config.h:9-25). */
+int pm_autotune(pm_graph* g, pm_pattern* const* p, int32_t n, const pm_opts* base,
+                const pm_tune_opts* t, pm_opts* best, pm_tune_report* report);
 
 #ifdef __cplusplus
 }

@@ -49,7 +49,17 @@ class Opts(C.Structure):
         ("package_bytes", C.c_uint32),
         ("prefetch", C.c_int32),
         ("order", C.c_int32),
+        ("sample", C.c_int32),
     ]
+
+
+class TuneOpts(C.Structure):
+    _fields_ = [("sample", C.c_int32), ("max_orders", C.c_int32), ("repeats", C.c_int32)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("base_ms", C.c_double), ("best_ms", C.c_double), ("trials", C.c_int32),
+                ("reordered", C.c_int32), ("sample", C.c_int32), ("pad", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -127,6 +137,8 @@ _SIGS = {
     "pm_match": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Result)]),
     "pm_match_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts),
                                  C.POINTER(Result)]),
+    "pm_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts),
+                              C.POINTER(TuneOpts), C.POINTER(Opts), C.POINTER(TuneReport)]),
 }
 
 _lib = None
@@ -311,7 +323,7 @@ POLICY = {"lru": 0, "fifo": 1, "learned": 2}
 
 
 def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
-               stream, cache_bytes, package_bytes, policy, prefetch, order="cost"):
+               stream, cache_bytes, package_bytes, policy, prefetch, order="cost", sample=0):
     o = Opts()
     lib().pm_opts_default(C.byref(o))
     o.waves, o.chunk, o.repeats = waves, chunk, repeats
@@ -320,6 +332,7 @@ def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge
     o.policy, o.prefetch = POLICY[policy], 1 if prefetch else 0
     o.edge_lo, o.edge_hi = edge_lo, edge_hi
     o.order = ORDER[order]
+    o.sample = sample
     pe = None
     if per_edge:
         pe = np.zeros(g.m, dtype=np.uint64)
@@ -341,12 +354,12 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
           split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
           per_edge: bool = False, stream: int | None = None, cache_bytes: int = 0,
           package_bytes: int = 0, policy: str = "lru", prefetch: bool = False,
-          order: str = "cost") -> MatchResult:
+          order: str = "cost", sample: int = 0) -> MatchResult:
     """Count matches of p in g on the GPU (pm_match).  order="file" matches
     labeled patterns in file order as the reference does; "cost" (default)
     lets the planner pick the order of vertices 2.. (same count)."""
     o, pe = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
-                       stream, cache_bytes, package_bytes, policy, prefetch, order)
+                       stream, cache_bytes, package_bytes, policy, prefetch, order, sample)
     r = Result()
     _check(lib().pm_match(g._h, p._h, C.byref(o), C.byref(r)), "match")
     return _result(r, pe)
@@ -355,18 +368,44 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
 def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: int = 1,
                 split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
                 stream: int | None = None, cache_bytes: int = 0, package_bytes: int = 0,
-                policy: str = "lru", prefetch: bool = False, order: str = "cost") -> list:
+                policy: str = "lru", prefetch: bool = False, order: str = "cost",
+                sample: int = 0) -> list:
     """Count several patterns in one persistent launch (pm_match_batch).
     Returns one MatchResult per pattern; count/jobs are the pattern's own,
     the timing fields describe the whole launch."""
     patterns = list(patterns)
     n = len(patterns)
     o, _ = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, False,
-                      stream, cache_bytes, package_bytes, policy, prefetch, order)
+                      stream, cache_bytes, package_bytes, policy, prefetch, order, sample)
     hs = (C.c_void_p * max(n, 1))(*[q._h.value for q in patterns])
     rs = (Result * max(n, 1))()
     _check(lib().pm_match_batch(g._h, hs, n, C.byref(o), rs), "match_batch")
     return [_result(rs[i]) for i in range(n)]
+
+
+def autotune(g: Graph, patterns, *, sample: int = 0, max_orders: int = 0, repeats: int = 0,
+             **base):
+    """pm_autotune: time sampled launches and return (options, report).
+    The options are a dict for match()/match_batch(); the patterns remember
+    their measured-fastest core order for this graph."""
+    patterns = list(patterns)
+    n = len(patterns)
+    o, _ = _make_opts(g, base.get("waves", 0), base.get("chunk", 0), 1, base.get("split", 0),
+                      base.get("hub_factor", 0), base.get("edge_lo", 0), base.get("edge_hi", 0),
+                      False, base.get("stream"), base.get("cache_bytes", 0),
+                      base.get("package_bytes", 0), base.get("policy", "lru"),
+                      base.get("prefetch", False), base.get("order", "cost"))
+    t = TuneOpts(sample, max_orders, repeats)
+    best = Opts()
+    rep = TuneReport()
+    hs = (C.c_void_p * max(n, 1))(*[q._h.value for q in patterns])
+    _check(lib().pm_autotune(g._h, hs, n, C.byref(o), C.byref(t), C.byref(best), C.byref(rep)),
+           "autotune")
+    opts = dict(base)
+    opts.update(chunk=best.chunk, split=best.split, hub_factor=best.hub_factor)
+    report = {"base_ms": rep.base_ms, "best_ms": rep.best_ms, "trials": rep.trials,
+              "reordered": rep.reordered, "sample": rep.sample}
+    return opts, report
 
 
 def pattern_text(k: int, edges, labels=None) -> str:

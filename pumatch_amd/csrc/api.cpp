@@ -253,4 +253,15 @@ int pm_match_batch(pm_graph* g, const pm_pattern* const* p, int32_t n, const pm_
   return device_match_batch(g, plans.data(), n, opts, results);
 }
 
+int pm_autotune(pm_graph* g, pm_pattern* const* p, int32_t n, const pm_opts* base,
+                const pm_tune_opts* t, pm_opts* best, pm_tune_report* report) {
+  if (!g || !p || n <= 0 || !best || !report) return PM_ERR_ARG;
+  pm_opts def;
+  pm_opts_default(&def);
+  if (!base) base = &def;
+  for (int32_t i = 0; i < n; i++)
+    if (!p[i]) return PM_ERR_ARG;
+  return autotune(g, p, n, base, t, best, report);
+}
+
 }  // extern "C"

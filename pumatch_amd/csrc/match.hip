@@ -1104,12 +1104,17 @@ __device__ __forceinline__ uint32_t source_full(const KGraph& g, uint64_t e) {
 // Phase-1 work units are (pattern, chunk of edges) pairs numbered pattern-
 // major and claimed from one cursor; tasks and deferred pieces carry their
 // pattern.
+constexpr uint64_t kSampleBlock = 4096;  // edges per block of a sampled run
+
 struct Launch {
   KGraph g;
   const KPlan* plans;
   uint32_t npat;
   uint32_t chunk;
   uint64_t nchunk;             // edge chunks per pattern
+  uint64_t npos;               // job positions per pattern (edges, or sampled edges)
+  uint32_t sample;             // > 1: only every sample-th block of kSampleBlock edges
+  uint32_t pad2;
   unsigned long long* cursor;  // [0] units [1] task claim [2] tasks [3] piece claim [4] pieces
   uint64_t edge_lo, edge_hi;
   uint32_t cap;
@@ -1126,6 +1131,12 @@ struct Launch {
   Resume* rout;  // deferred pieces for the next round
   uint64_t rout_cap;
 };
+
+// edge of job position s (positions enumerate the sampled edges in order)
+__device__ __forceinline__ uint64_t edge_at(const Launch& L, uint64_t s) {
+  if (L.sample <= 1) return L.edge_lo + s;
+  return L.edge_lo + (s / kSampleBlock) * (kSampleBlock * L.sample) + s % kSampleBlock;
+}
 
 __device__ __forceinline__ DeferOut defer_out(const Launch& L) {
   DeferOut O;
@@ -1171,12 +1182,12 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
     }
     __syncthreads();
     const KPlan& P = L.plans[p];
-    const uint64_t e0 = L.edge_lo + (u % L.nchunk) * L.chunk;
-    const uint64_t e1 = min(e0 + L.chunk, L.edge_hi);
-    uint32_t rlo = uni(source_full(g, e0));
-    for (uint64_t eb = e0; eb < e1; eb += kWave) {
-      const uint64_t e = eb + lane;
-      bool ok = e < e1;
+    const uint64_t s0 = (u % L.nchunk) * L.chunk;
+    const uint64_t s1 = min(s0 + L.chunk, L.npos);
+    uint32_t rlo = uni(source_full(g, edge_at(L, s0)));
+    for (uint64_t sb = s0; sb < s1; sb += kWave) {
+      bool ok = sb + lane < s1;
+      const uint64_t e = ok ? edge_at(L, sb + lane) : 0;
       uint32_t r = rlo, c = 0;
       if (ok) {
         r = source_of(g, e, rlo);
@@ -1185,25 +1196,26 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
         if (ok && P.labeled) ok = g.lab[r] == P.lab0 && g.lab[c] == P.lab1;
         if (ok) ok = degree(g, r) >= P.deg0 && degree(g, c) >= P.deg1 && g.maxnbr[c] >= P.deg2;
       }
-      const uint32_t lastl = (uint32_t)min((uint64_t)kWave, e1 - eb) - 1u;
+      const uint32_t lastl = (uint32_t)min((uint64_t)kWave, s1 - sb) - 1u;
       uint64_t jm = __ballot(ok);
       ws.jr[lane] = r;
       ws.jc[lane] = c;
       __syncthreads();
       rlo = uni((uint32_t)__shfl((int)r, (int)lastl));
-      ct.alg += 24ull * min((uint64_t)kWave, e1 - eb);
+      ct.alg += 24ull * min((uint64_t)kWave, s1 - sb);
       jm = uni64(jm);
       while (jm) {
         const int i = __builtin_ctzll(jm);
         jm &= jm - 1;
         const uint32_t jr = uni(ws.jr[i]), jc = uni(ws.jc[i]);
+        const uint64_t ej = uni64(__shfl((unsigned long long)e, i));
         const unsigned long long cnt =
-            run_job<PG>(g, P, ws, arena, L.cap, lane, jr, jc, eb + i, 0, kInf, false,
+            run_job<PG>(g, P, ws, arena, L.cap, lane, jr, jc, ej, 0, kInf, false,
                         L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
         if (lane == 0) {
           ws.acc_count += cnt;
           ws.acc_jobs++;
-          if (L.per_edge && cnt) atomicAdd(&L.per_edge[eb + i - L.edge_lo], cnt);
+          if (L.per_edge && cnt) atomicAdd(&L.per_edge[ej - L.edge_lo], cnt);
         }
       }
       __syncthreads();
@@ -1824,8 +1836,12 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     for (int q = 0; q < npat; q++)
       if (plans[q]->labeled && plans[q]->kc >= 4) {
         reordered.push_back(*plans[q]);
-        reorder_core(&reordered.back(), g->stats);
-        use[q] = &reordered.back();
+        Plan& pl = reordered.back();
+        if (pl.tuned_graph == (const void*)g)
+          (void)apply_core_order(&pl, std::vector<int>(pl.tuned_core + 2, pl.tuned_core + pl.kc));
+        else
+          reorder_core(&pl, g->stats);
+        use[q] = &pl;
       }
   }
   plans = use.data();
@@ -1851,12 +1867,20 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     set_error("edge range outside the graph");
     return PM_ERR_ARG;
   }
+  // sampled runs (autotuning): every sample-th block of kSampleBlock edges
+  const uint32_t sample = opts->sample > 1 ? (uint32_t)opts->sample : 1u;
+  uint64_t npos = hi - lo;
+  if (sample > 1) {
+    const uint64_t stride = kSampleBlock * sample;
+    const uint64_t full = (hi - lo) / stride;
+    npos = full * kSampleBlock + std::min<uint64_t>(kSampleBlock, (hi - lo) - full * stride);
+  }
   const uint32_t chunk =
       opts->chunk > 0
           ? (uint32_t)opts->chunk
           : (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(64, (hi - lo) * npat / ((uint64_t)waves * 8)));
-  const uint64_t nchunk = (hi - lo + chunk - 1) / chunk;
+                1, std::min<uint64_t>(64, npos * npat / ((uint64_t)waves * 8)));
+  const uint64_t nchunk = (npos + chunk - 1) / chunk;
   for (KPlan& kp : kps) {
     kp.split = opts->split > 0 ? (uint32_t)opts->split : 8192u;
     kp.hub_factor = opts->hub_factor > 0 ? (uint32_t)opts->hub_factor : 8u;
@@ -1956,6 +1980,8 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   L.plans = (const KPlan*)g_ws.plans;
   L.npat = (uint32_t)npat;
   L.nchunk = nchunk;
+  L.npos = npos;
+  L.sample = sample;
   L.pcount = (unsigned long long*)g_ws.pcount;
   L.cursor = g_ws.cursor;
   L.edge_lo = lo;

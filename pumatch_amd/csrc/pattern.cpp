@@ -267,29 +267,33 @@ int build_core(Plan* pp, std::string* err) {
   return PM_OK;
 }
 
-// Labeled plans: choose the order of core vertices 2..kc-1.  A labeled
-// count is the number of homomorphisms with phi(0) = r, phi(1) = c summed
-// over the jobs, so it does not depend on the order in which vertices 2..
-// are matched (the job filter keeps the file-order degrees, see deg[]);
-// only the work does.  Among all orders in which every vertex has an
-// earlier neighbour, take the one with the fewest estimated partial matches
-// at the enumerated depths 2..kc-2 (the last vertex is counted, not
-// enumerated).  A vertex with b earlier neighbours and label l has about
+// Labeled plans: the order of core vertices 2..kc-1.  A labeled count is
+// the number of homomorphisms with phi(0) = r, phi(1) = c summed over the
+// jobs, so it does not depend on the order in which vertices 2.. are
+// matched (the job filter keeps the file-order degrees, see deg[]); only the
+// work does.  Candidate orders are those in which every vertex has an
+// earlier neighbour.  The cost model estimates the partial matches at the
+// enumerated depths 2..kc-2 (the last vertex is counted, not enumerated): a
+// vertex with b earlier neighbours and label l has about
 // dnb * f(l) * q^(b-1) candidates, dnb the mean neighbour degree, f(l) the
 // label's share and q the chance that a candidate also lies in one more
-// neighbour list.  Ties keep file order.
-void reorder_core(Plan* pp, const OrderStats& st) {
-  Plan& p = *pp;
-  if (!p.labeled || p.kc < 4) return;
+// neighbour list.  Returns up to `max` orders (core vertex ids at positions
+// 2..kc-1), cheapest first; the file order is always included.
+std::vector<std::vector<int>> rank_core_orders(const Plan& p, const OrderStats& st, int max) {
+  std::vector<std::vector<int>> out;
   const int kc = p.kc;
+  std::vector<int> file(p.core_v + 2, p.core_v + std::max(kc, 2));
+  if (!p.labeled || kc < 4) {
+    out.push_back(file);
+    return out;
+  }
   const double n = std::max(st.n, 1.0);
   const double dnb = std::max(st.dnb, 1.0);
   const double q = std::min(0.5, std::max(0.02, 4.0 * dnb / n));
   auto share = [&](int l) {
     return (l >= 0 && l < (int)st.label_share.size()) ? std::max(st.label_share[l], 1e-6) : 1e-6;
   };
-  std::vector<int> rest(p.core_v + 2, p.core_v + kc), best = rest;
-  auto cost_of = [&](const std::vector<int>& ord, double* out) {
+  auto cost_of = [&](const std::vector<int>& ord, double* out_cost) {
     std::vector<int> seq = {p.core_v[0], p.core_v[1]};
     seq.insert(seq.end(), ord.begin(), ord.end());
     double prefixes = 1.0, cost = 0.0;
@@ -303,23 +307,51 @@ void reorder_core(Plan* pp, const OrderStats& st) {
         cost += prefixes;
       }
     }
-    *out = cost;
+    *out_cost = cost;
     return true;
   };
-  double best_cost = 0;
-  if (!cost_of(best, &best_cost)) return;  // file order is always valid
+  std::vector<std::pair<double, std::vector<int>>> all;
+  double file_cost = 0;
+  if (!cost_of(file, &file_cost)) file_cost = 1e300;
+  std::vector<int> rest = file;
   std::sort(rest.begin(), rest.end());
   do {
     double c;
-    if (cost_of(rest, &c) && c < best_cost * 0.999) {
-      best_cost = c;
-      best = rest;
-    }
+    if (rest != file && cost_of(rest, &c)) all.push_back({c, rest});
   } while (std::next_permutation(rest.begin(), rest.end()));
-  const Plan file_order = p;
-  for (int i = 2; i < kc; i++) p.core_v[i] = best[i - 2];
+  // ties keep file order first
+  all.push_back({file_cost * 0.999, file});
+  std::stable_sort(all.begin(), all.end(),
+                   [](const auto& a, const auto& b) { return a.first < b.first; });
+  max = std::max(max, 1);
+  bool have_file = false;
+  for (size_t i = 0; i < all.size() && (int)out.size() < max; i++) {
+    out.push_back(all[i].second);
+    have_file |= all[i].second == file;
+  }
+  if (!have_file) out.back() = file;
+  return out;
+}
+
+// Rebuild p's core for the order `rest` (core vertex ids at positions
+// 2..kc-1); false (p unchanged) when that order needs too many slots.
+bool apply_core_order(Plan* pp, const std::vector<int>& rest) {
+  Plan& p = *pp;
+  if ((int)rest.size() != std::max(p.kc - 2, 0)) return false;
+  const Plan before = p;
+  for (int i = 2; i < p.kc; i++) p.core_v[i] = rest[i - 2];
   std::string err;
-  if (build_core(&p, &err) != PM_OK) p = file_order;
+  if (build_core(&p, &err) != PM_OK) {
+    p = before;
+    return false;
+  }
+  return true;
+}
+
+void reorder_core(Plan* p, const OrderStats& st) {
+  if (!p->labeled || p->kc < 4) return;
+  for (const auto& o : rank_core_orders(*p, st, 2))
+    if (apply_core_order(p, o)) return;
 }
 
 OrderStats order_stats(const HostGraph& h) {

@@ -114,6 +114,11 @@ struct Plan {
   int slot_of[kMaxK][kMaxK];     // [depth][u] slot holding u's running set
   std::vector<SlotOp> ops[kMaxK];  // ops run at depth d (d = 0..kc-2)
   int n_materialised = 0;        // INTERSECT slots (arena sizing)
+
+  // core order measured fastest by pm_autotune on graph `tuned_graph`
+  // (positions 2..kc-1); used there instead of the cost model
+  const void* tuned_graph = nullptr;
+  int tuned_core[kMaxK] = {};
 };
 
 }  // namespace pm
@@ -130,6 +135,8 @@ int build_plan(Plan* p, std::string* err);
 int build_core(Plan* p, std::string* err);
 // labeled plans: cost-based order of core vertices 2..kc-1 (same count)
 void reorder_core(Plan* p, const OrderStats& st);
+std::vector<std::vector<int>> rank_core_orders(const Plan& p, const OrderStats& st, int max);
+bool apply_core_order(Plan* p, const std::vector<int>& rest);
 OrderStats order_stats(const HostGraph& h);
 
 // generators (gen.cpp)
@@ -154,5 +161,7 @@ int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* 
 // and the launch-wide timings.
 int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm_opts* opts,
                        pm_result* res);
+int autotune(pm_graph* g, pm_pattern* const* p, int n, const pm_opts* base,
+             const pm_tune_opts* t, pm_opts* best, pm_tune_report* rep);
 
 }  // namespace pm

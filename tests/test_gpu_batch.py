@@ -95,3 +95,30 @@ def test_batch_per_edge_rejected(rmat10):
     hs = (pm.C.c_void_p * 2)(p._h.value, p._h.value)
     rs = (pm.Result * 2)()
     assert pm.lib().pm_match_batch(rmat10._h, hs, 2, pm.C.byref(o), rs) == pm.PM_ERR_ARG
+
+
+def test_sampled_run_counts_sampled_jobs(rmat10):
+    """A sampled launch (every s-th block of 4096 edges) counts exactly the
+    jobs of those blocks: the sum of the full run's per-edge counts there."""
+    g = pm.Graph.rmat(13, 8, seed=5)
+    g.set_uniform_labels(4, 7)
+    text = batch_patterns(3, seed=6, nlabels=4, kmin=5, kmax=5)[2]
+    p = pm.Pattern.parse(text, True)
+    full = pm.match(g, p, per_edge=True)
+    for s in (2, 3, 7):
+        mask = (np.arange(g.m) // 4096) % s == 0
+        want = int(full.per_edge[mask].sum())
+        print(f"[sample] s={s}", flush=True)
+        assert pm.match(g, p, sample=s).count == want
+        assert pm.match_batch(g, [p, p], sample=s)[1].count == want
+
+
+def test_autotune_keeps_counts(rmat10):
+    texts = mixed_set()[:12]
+    pats = [pm.Pattern.parse(t, lab) for t, lab in texts]
+    base = [r.count for r in pm.match_batch(rmat10, pats, order="file")]
+    opts, rep = pm.autotune(rmat10, pats, max_orders=3)
+    print(f"[autotune] {opts} {rep}", flush=True)
+    assert rep["trials"] > 0 and rep["best_ms"] > 0
+    assert [r.count for r in pm.match_batch(rmat10, pats, **opts)] == base
+    assert [pm.match(rmat10, p, **opts).count for p in pats] == base
