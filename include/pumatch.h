@@ -74,7 +74,16 @@ typedef struct {
   int32_t sample;       /* > 1: jobs only from every sample-th block of 4096
                            CSR edges (timing samples; the count is then that
                            of the sampled jobs) */
+  int32_t learned;      /* PM_LEARN_* bits, used when model is set */
+  const struct pm_model* model;
+  uint64_t* root_work;  /* trace, optional host u64[n]: work units (algorithmic
+                           bytes) of the jobs rooted at each vertex */
+  uint32_t* package_use; /* trace, optional host u32[n]: out-of-core rounds in
+                           which each vertex's package was used (0 resident) */
 } pm_opts;
+
+#define PM_LEARN_ORDER 0x1  /* phase-1 jobs heaviest predicted root first */
+#define PM_LEARN_PAGING 0x2 /* predicted package use ranks eviction/prefetch */
 
 typedef struct {
   double kernel_ms;     /* mean kernel time (hipEvents on the launch stream) */
@@ -188,6 +197,49 @@ int pm_match(pm_graph* g, const pm_pattern* p, const pm_opts* opts, pm_result* r
  * pattern file (plot/plot_fig11.sh:112-123).  opts->per_edge needs n == 1. */
 int pm_match_batch(pm_graph* g, const pm_pattern* const* p, int32_t n, const pm_opts* opts,
                    pm_result* results);
+
+/* -- learned scheduling and paging ----------------------------------------
+ * A 16 -> 64 -> 64 -> 2 MLP over per-vertex features.  Output 0 predicts
+ * log2(1 + work of the jobs rooted at the vertex), output 1 log2(1 + rounds
+ * in which the vertex's package was used); their training targets are the
+ * traces pm_opts.root_work / package_use record.  With pm_opts.model set,
+ * PM_LEARN_ORDER takes phase-1 jobs heaviest predicted root first and
+ * PM_LEARN_PAGING ranks packages for eviction (PM_POLICY_LEARNED) and
+ * prefetch.  No counterpart in the reference (CSR-order job queue,
+ * src/job_queue.h:39-89; LRU buffer, src/buffer.cu:581-796); counts never
+ * depend on the model. */
+#define PM_FEATURES 16
+#define PM_MODEL_PARAMS 5378 /* W1[16][64] b1[64] W2[64][64] b2[64] W3[64][2] b3[2] */
+
+typedef struct pm_model pm_model;
+
+typedef struct {
+  int32_t epochs;  /* passes over the samples (0 = 20) */
+  int32_t batch;   /* samples per Adam step (0 = 4096) */
+  float lr;        /* Adam step size (0 = 3e-3) */
+  int32_t pad;
+  uint64_t seed;   /* sample order (0 = 1) */
+} pm_train_opts;
+
+/* He-uniform initial weights from `seed` (FP32 master copy on the host). */
+int pm_model_create(uint64_t seed, pm_model** out);
+void pm_model_destroy(pm_model* m);
+/* copy the PM_MODEL_PARAMS FP32 parameters out / in (layout above,
+ * row-major [in][out]) */
+int pm_model_get(const pm_model* m, float* params);
+int pm_model_set(pm_model* m, const float* params);
+/* features of every vertex of g on the GPU: out = host float[n][16] */
+int pm_graph_features(pm_graph* g, float* out);
+/* inference with BF16 weights and inputs on MFMA, FP32 accumulation:
+ * X float[n][16] -> Y float[n][2] */
+int pm_model_predict(const pm_model* m, const float* X, uint64_t n, float* Y);
+/* FP32 loss sum_{s,o} W[s][o] (yhat - y)^2 / sum W and its gradient with
+ * respect to the parameters (W NULL: all ones) */
+int pm_model_grad(const pm_model* m, const float* X, const float* Y, const float* W, uint64_t n,
+                  float* grad, double* loss);
+/* Adam on the GPU (FP32 forward/backward kernels) */
+int pm_model_train(pm_model* m, const float* X, const float* Y, const float* W, uint64_t n,
+                   const pm_train_opts* o, double* final_loss);
 
 /* -- autotuning ------------------------------------------------------------ */
 

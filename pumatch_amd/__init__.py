@@ -50,7 +50,16 @@ class Opts(C.Structure):
         ("prefetch", C.c_int32),
         ("order", C.c_int32),
         ("sample", C.c_int32),
+        ("learned", C.c_int32),
+        ("model", C.c_void_p),
+        ("root_work", C.POINTER(C.c_uint64)),
+        ("package_use", C.POINTER(C.c_uint32)),
     ]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
+                ("pad", C.c_int32), ("seed", C.c_uint64)]
 
 
 class TuneOpts(C.Structure):
@@ -137,6 +146,19 @@ _SIGS = {
     "pm_match": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Result)]),
     "pm_match_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts),
                                  C.POINTER(Result)]),
+    "pm_model_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pm_model_destroy": (None, [C.c_void_p]),
+    "pm_model_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_model_set": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_graph_features": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_model_predict": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint64,
+                                   C.POINTER(C.c_float)]),
+    "pm_model_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_float),
+                                C.POINTER(C.c_double)]),
+    "pm_model_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float), C.c_uint64, C.POINTER(TrainOpts),
+                                 C.POINTER(C.c_double)]),
     "pm_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts),
                               C.POINTER(TuneOpts), C.POINTER(Opts), C.POINTER(TuneReport)]),
 }
@@ -316,14 +338,89 @@ class MatchResult:
     packages: int = 0
     link_ms: float = 0.0
     per_edge: np.ndarray | None = None
+    root_work: np.ndarray | None = None
+    package_use: np.ndarray | None = None
 
 
 ORDER = {"cost": 0, "file": 1}
+PM_LEARN_ORDER, PM_LEARN_PAGING = 0x1, 0x2
+N_FEATURES, N_PARAMS = 16, 5378
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Model:
+    """The 16 -> 64 -> 64 -> 2 MLP (pm_model)."""
+
+    def __init__(self, seed: int = 1):
+        h = C.c_void_p()
+        _check(lib().pm_model_create(seed, C.byref(h)), "model_create")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().pm_model_destroy(self._h)
+            self._h = C.c_void_p()
+
+    @property
+    def params(self) -> np.ndarray:
+        w = np.zeros(N_PARAMS, np.float32)
+        _check(lib().pm_model_get(self._h, _fptr(w)), "model_get")
+        return w
+
+    @params.setter
+    def params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == N_PARAMS
+        _check(lib().pm_model_set(self._h, _fptr(w)), "model_set")
+
+    def predict(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, N_FEATURES)
+        Y = np.zeros((X.shape[0], 2), np.float32)
+        _check(lib().pm_model_predict(self._h, _fptr(X), X.shape[0], _fptr(Y)), "model_predict")
+        return Y
+
+    def grad(self, X, Y, W=None):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, N_FEATURES)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(-1, 2)
+        Wp = None
+        if W is not None:
+            W = np.ascontiguousarray(W, dtype=np.float32).reshape(-1, 2)
+            Wp = _fptr(W)
+        g = np.zeros(N_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(lib().pm_model_grad(self._h, _fptr(X), _fptr(Y), Wp, X.shape[0], _fptr(g),
+                                   C.byref(loss)), "model_grad")
+        return loss.value, g
+
+    def train(self, X, Y, W=None, *, epochs: int = 0, batch: int = 0, lr: float = 0.0,
+              seed: int = 0) -> float:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, N_FEATURES)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(-1, 2)
+        Wp = None
+        if W is not None:
+            W = np.ascontiguousarray(W, dtype=np.float32).reshape(-1, 2)
+            Wp = _fptr(W)
+        t = TrainOpts(epochs, batch, lr, 0, seed)
+        loss = C.c_double()
+        _check(lib().pm_model_train(self._h, _fptr(X), _fptr(Y), Wp, X.shape[0], C.byref(t),
+                                    C.byref(loss)), "model_train")
+        return loss.value
+
+
+def features(g: "Graph") -> np.ndarray:
+    """Per-vertex features (n x 16) computed on the GPU (pm_graph_features)."""
+    out = np.zeros((max(g.n, 1), N_FEATURES), np.float32)
+    _check(lib().pm_graph_features(g._h, _fptr(out)), "graph_features")
+    return out[:g.n]
 POLICY = {"lru": 0, "fifo": 1, "learned": 2}
 
 
 def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
-               stream, cache_bytes, package_bytes, policy, prefetch, order="cost", sample=0):
+               stream, cache_bytes, package_bytes, policy, prefetch, order="cost", sample=0,
+               model=None, learned=0):
     o = Opts()
     lib().pm_opts_default(C.byref(o))
     o.waves, o.chunk, o.repeats = waves, chunk, repeats
@@ -333,6 +430,9 @@ def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge
     o.edge_lo, o.edge_hi = edge_lo, edge_hi
     o.order = ORDER[order]
     o.sample = sample
+    if model is not None:
+        o.model = model._h.value
+        o.learned = learned if learned else (PM_LEARN_ORDER | PM_LEARN_PAGING)
     pe = None
     if per_edge:
         pe = np.zeros(g.m, dtype=np.uint64)
@@ -354,29 +454,45 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
           split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
           per_edge: bool = False, stream: int | None = None, cache_bytes: int = 0,
           package_bytes: int = 0, policy: str = "lru", prefetch: bool = False,
-          order: str = "cost", sample: int = 0) -> MatchResult:
+          order: str = "cost", sample: int = 0, model=None, learned: int = 0,
+          traces: bool = False) -> MatchResult:
     """Count matches of p in g on the GPU (pm_match).  order="file" matches
     labeled patterns in file order as the reference does; "cost" (default)
-    lets the planner pick the order of vertices 2.. (same count)."""
+    lets the planner pick the order of vertices 2.. (same count).  With a
+    Model, `learned` (PM_LEARN_* bits, default both) turns on learned job
+    order and package ranking.  traces=True records root_work and
+    package_use (both per vertex) into the result."""
     o, pe = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
-                       stream, cache_bytes, package_bytes, policy, prefetch, order, sample)
+                       stream, cache_bytes, package_bytes, policy, prefetch, order, sample,
+                       model, learned)
+    rw = pu = None
+    if traces:
+        rw = np.zeros(max(g.n, 1), dtype=np.uint64)
+        pu = np.zeros(max(g.n, 1), dtype=np.uint32)
+        o.root_work = rw.ctypes.data_as(C.POINTER(C.c_uint64))
+        o.package_use = pu.ctypes.data_as(C.POINTER(C.c_uint32))
     r = Result()
     _check(lib().pm_match(g._h, p._h, C.byref(o), C.byref(r)), "match")
-    return _result(r, pe)
+    res = _result(r, pe)
+    if traces:
+        res.root_work = rw[:g.n]
+        res.package_use = pu[:g.n]
+    return res
 
 
 def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: int = 1,
                 split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
                 stream: int | None = None, cache_bytes: int = 0, package_bytes: int = 0,
                 policy: str = "lru", prefetch: bool = False, order: str = "cost",
-                sample: int = 0) -> list:
+                sample: int = 0, model=None, learned: int = 0) -> list:
     """Count several patterns in one persistent launch (pm_match_batch).
     Returns one MatchResult per pattern; count/jobs are the pattern's own,
     the timing fields describe the whole launch."""
     patterns = list(patterns)
     n = len(patterns)
     o, _ = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, False,
-                      stream, cache_bytes, package_bytes, policy, prefetch, order, sample)
+                      stream, cache_bytes, package_bytes, policy, prefetch, order, sample,
+                      model, learned)
     hs = (C.c_void_p * max(n, 1))(*[q._h.value for q in patterns])
     rs = (Result * max(n, 1))()
     _check(lib().pm_match_batch(g._h, hs, n, C.byref(o), rs), "match_batch")

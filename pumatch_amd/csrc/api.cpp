@@ -1,4 +1,5 @@
 // C ABI of libpumatch.so (declared in include/pumatch.h).
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <new>
@@ -251,6 +252,55 @@ int pm_match_batch(pm_graph* g, const pm_pattern* const* p, int32_t n, const pm_
     plans[i] = &p[i]->plan;
   }
   return device_match_batch(g, plans.data(), n, opts, results);
+}
+
+int pm_model_create(uint64_t seed, pm_model** out) {
+  if (!out) return PM_ERR_ARG;
+  pm_model* m = new (std::nothrow) pm_model();
+  if (!m) return PM_ERR_NOMEM;
+  model_init(m, seed);
+  *out = m;
+  return PM_OK;
+}
+
+void pm_model_destroy(pm_model* m) { delete m; }
+
+int pm_model_get(const pm_model* m, float* params) {
+  if (!m || !params) return PM_ERR_ARG;
+  std::memcpy(params, m->w.data(), sizeof(float) * PM_MODEL_PARAMS);
+  return PM_OK;
+}
+
+int pm_model_set(pm_model* m, const float* params) {
+  if (!m || !params) return PM_ERR_ARG;
+  std::memcpy(m->w.data(), params, sizeof(float) * PM_MODEL_PARAMS);
+  std::fill(m->adam_m.begin(), m->adam_m.end(), 0.0f);
+  std::fill(m->adam_v.begin(), m->adam_v.end(), 0.0f);
+  m->step = 0;
+  m->version = next_model_version();
+  return PM_OK;
+}
+
+int pm_graph_features(pm_graph* g, float* out) {
+  if (!g || (!out && g->h.n)) return PM_ERR_ARG;
+  return graph_features_host(g, out);
+}
+
+int pm_model_predict(const pm_model* m, const float* X, uint64_t n, float* Y) {
+  if (!m || (n && (!X || !Y))) return PM_ERR_ARG;
+  return model_predict_host(m, X, n, Y);
+}
+
+int pm_model_grad(const pm_model* m, const float* X, const float* Y, const float* W, uint64_t n,
+                  float* grad, double* loss) {
+  if (!m || !grad || (n && (!X || !Y))) return PM_ERR_ARG;
+  return model_grad(m, X, Y, W, n, grad, loss);
+}
+
+int pm_model_train(pm_model* m, const float* X, const float* Y, const float* W, uint64_t n,
+                   const pm_train_opts* o, double* final_loss) {
+  if (!m || (n && (!X || !Y))) return PM_ERR_ARG;
+  return model_train(m, X, Y, W, n, o, final_loss);
 }
 
 int pm_autotune(pm_graph* g, pm_pattern* const* p, int32_t n, const pm_opts* base,

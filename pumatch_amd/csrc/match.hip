@@ -55,6 +55,7 @@
 #include <memory>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -172,6 +173,8 @@ struct WaveState {
   uint32_t cval[kMaxDepth][kWave];  // depths 1..k-3 <= 4
   uint32_t jr[kWave];
   uint32_t jc[kWave];
+  unsigned long long jedge[kWave];  // CSR edge of each lane's job
+  unsigned long long alg0;          // work counter at the start of the job (trace)
   // fused step: per-owner data of the current batch
   uint32_t own_off[kWave + 1];
   uint32_t own_ub[kWave];
@@ -1096,6 +1099,22 @@ __device__ __forceinline__ uint32_t source_of(const KGraph& g, uint64_t e, uint3
 }
 
 // source of edge e: start at the indexed source of edge 64*(e/64), gallop
+// index i of the last starts[i] <= s, galloping up from lo
+__device__ __forceinline__ uint32_t gallop(const uint64_t* starts, uint32_t n, uint64_t s,
+                                           uint32_t lo) {
+  uint32_t step = 1;
+  while (lo + step < n && starts[lo + step] <= s) {
+    lo += step;
+    step <<= 1;
+  }
+  uint32_t hi = min(lo + step, n);
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (starts[mid] <= s) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ uint32_t source_full(const KGraph& g, uint64_t e) {
   return source_of(g, e, g.src64[e >> 6]);
 }
@@ -1130,6 +1149,11 @@ struct Launch {
   uint64_t nrin;
   Resume* rout;  // deferred pieces for the next round
   uint64_t rout_cap;
+  // learned job order: position s lies in the row of root_order[i] where
+  // root_start[i] <= s < root_start[i+1] (null: CSR order)
+  const uint32_t* root_order;
+  const uint64_t* root_start;
+  unsigned long long* root_work;  // trace: work of the jobs rooted at each vertex
 };
 
 // edge of job position s (positions enumerate the sampled edges in order)
@@ -1184,13 +1208,23 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
     const KPlan& P = L.plans[p];
     const uint64_t s0 = (u % L.nchunk) * L.chunk;
     const uint64_t s1 = min(s0 + L.chunk, L.npos);
-    uint32_t rlo = uni(source_full(g, edge_at(L, s0)));
+    const bool lorder = L.root_order != nullptr;
+    // rlo: source vertex (CSR order) or root index (learned order) of s0
+    uint32_t rlo = lorder ? uni(gallop(L.root_start, (uint32_t)g.n, s0, 0))
+                          : uni(source_full(g, edge_at(L, s0)));
     for (uint64_t sb = s0; sb < s1; sb += kWave) {
       bool ok = sb + lane < s1;
-      const uint64_t e = ok ? edge_at(L, sb + lane) : 0;
-      uint32_t r = rlo, c = 0;
+      uint64_t e = 0;
+      uint32_t r = rlo, c = 0, ri = rlo;
+      if (ok && lorder) {
+        ri = gallop(L.root_start, (uint32_t)g.n, sb + lane, rlo);
+        r = L.root_order[ri];
+        e = g.rowptr[r] + (sb + lane - L.root_start[ri]);
+      } else if (ok) {
+        e = edge_at(L, sb + lane);
+      }
       if (ok) {
-        r = source_of(g, e, rlo);
+        if (!lorder) r = source_of(g, e, rlo);
         c = g.jcol[e];
         if (!P.labeled && P.rc_order && !(r > c)) ok = false;
         if (ok && P.labeled) ok = g.lab[r] == P.lab0 && g.lab[c] == P.lab1;
@@ -1200,18 +1234,22 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
       uint64_t jm = __ballot(ok);
       ws.jr[lane] = r;
       ws.jc[lane] = c;
+      ws.jedge[lane] = e;
       __syncthreads();
-      rlo = uni((uint32_t)__shfl((int)r, (int)lastl));
+      rlo = uni((uint32_t)__shfl((int)(lorder ? ri : r), (int)lastl));
       ct.alg += 24ull * min((uint64_t)kWave, s1 - sb);
       jm = uni64(jm);
       while (jm) {
         const int i = __builtin_ctzll(jm);
         jm &= jm - 1;
         const uint32_t jr = uni(ws.jr[i]), jc = uni(ws.jc[i]);
-        const uint64_t ej = uni64(__shfl((unsigned long long)e, i));
+        const uint64_t ej = uni64(ws.jedge[i]);
+        if (lane == 0) ws.alg0 = ct.alg;
         const unsigned long long cnt =
             run_job<PG>(g, P, ws, arena, L.cap, lane, jr, jc, ej, 0, kInf, false,
                         L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
+        if (L.root_work && lane == 0)
+          atomicAdd(&L.root_work[jr], (unsigned long long)(ct.alg - ws.alg0));
         if (lane == 0) {
           ws.acc_count += cnt;
           ws.acc_jobs++;
@@ -1256,9 +1294,12 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_tasks(Launch L) {
     const uint32_t c = uni(L.tasks[t].c);
     const uint32_t p = uni(L.tasks[t].pat);
     ws.pat = p;
+    if (lane == 0) ws.alg0 = ct.alg;
     const unsigned long long cnt =
         run_job<PG>(g, L.plans[p], ws, arena, L.cap, lane, r, c, e, tlo, thi, false, false,
                     nullptr, nullptr, 0, O, ct);
+    if (L.root_work && lane == 0)
+      atomicAdd(&L.root_work[r], (unsigned long long)(ct.alg - ws.alg0));
     if (lane == 0 && cnt) atomicAdd(&L.pcount[2 * p], cnt);
     if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
   }
@@ -1292,9 +1333,12 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_resume(Launch L) {
     for (int d = 1; d <= depth && d < kMaxDepth; d++) ws.ridx[d] = uni(L.rin[t].idx[d]);
     for (int d = 0; d <= depth && d < kMaxDepth; d++) ws.phi[d] = uni(L.rin[t].phi[d]);
     ws.rend = uni(L.rin[t].end);
+    if (lane == 0) ws.alg0 = ct.alg;
     const unsigned long long cnt =
         run_job<PG>(g, L.plans[p], ws, arena, L.cap, lane, r, c, e, rlo, rhi, true,
                     depth == 0 && L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
+    if (L.root_work && lane == 0)
+      atomicAdd(&L.root_work[r], (unsigned long long)(ct.alg - ws.alg0));
     if (lane == 0 && cnt) atomicAdd(&L.pcount[2 * p], cnt);
     if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
   }
@@ -1626,6 +1670,10 @@ void device_release(pm_graph* g) {
   if (g->d.src64) (void)hipFree(g->d.src64);
   if (g->d.lcol) (void)hipFree(g->d.lcol);
   if (g->d.loff) (void)hipFree(g->d.loff);
+  if (g->d_root_order) (void)hipFree(g->d_root_order);
+  if (g->d_root_start) (void)hipFree(g->d_root_start);
+  g->d_root_order = nullptr;
+  g->d_root_start = nullptr;
   g->d = DeviceGraph();
   g->on_device = false;
 }
@@ -1780,6 +1828,42 @@ int load_packages(PagedGraph* pg, const std::vector<uint32_t>& want, std::vector
   return PM_OK;
 }
 
+// Learned job order for model m: roots by predicted work (output 0),
+// heaviest first (bucketed, ties by id), and the first job position of each.
+int learned_order(pm_graph* g, const pm_model* m, const std::vector<float>& pred) {
+  if (g->d_root_order && g->order_version == m->version) return PM_OK;
+  const HostGraph& h = g->h;
+  const uint64_t n = h.n;
+  float lo = 0, hi = 0;
+  for (uint64_t v = 0; v < n; v++) {
+    const float y = pred[2 * v];
+    if (v == 0 || y < lo) lo = y;
+    if (v == 0 || y > hi) hi = y;
+  }
+  constexpr int kBuckets = 4096;
+  const float scale = hi > lo ? (kBuckets - 1) / (hi - lo) : 0.0f;
+  std::vector<uint32_t> bucket(n), cnt(kBuckets + 1, 0);
+  for (uint64_t v = 0; v < n; v++) {
+    const int b = (kBuckets - 1) - std::min(kBuckets - 1, std::max(0, (int)((pred[2 * v] - lo) * scale)));
+    bucket[v] = (uint32_t)b;
+    cnt[b + 1]++;
+  }
+  for (int b = 0; b < kBuckets; b++) cnt[b + 1] += cnt[b];
+  std::vector<uint32_t> order(n);
+  for (uint64_t v = 0; v < n; v++) order[cnt[bucket[v]]++] = (uint32_t)v;
+  std::vector<uint64_t> start(n + 1, 0);
+  for (uint64_t i = 0; i < n; i++)
+    start[i + 1] = start[i] + (h.rowptr[order[i] + 1] - h.rowptr[order[i]]);
+  if (!g->d_root_order) {
+    HIP_OK(hipMalloc(&g->d_root_order, sizeof(uint32_t) * std::max<uint64_t>(n, 1)));
+    HIP_OK(hipMalloc(&g->d_root_start, sizeof(uint64_t) * (n + 1)));
+  }
+  if (n) HIP_OK(hipMemcpy(g->d_root_order, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(g->d_root_start, start.data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice));
+  g->order_version = m->version;
+  return PM_OK;
+}
+
 int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res) {
   const Plan* one = &plan;
   return device_match_batch(g, &one, 1, opts, res);
@@ -1804,6 +1888,13 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   const bool paged = opts->cache_bytes > 0;
   int rc = device_upload_rows(g, !paged);
   if (rc) return rc;
+  // learned parts: model predictions per vertex (cached per graph and model)
+  const pm_model* model = opts->model;
+  const std::vector<float>* pred = nullptr;
+  if (model && (opts->learned & (PM_LEARN_ORDER | PM_LEARN_PAGING))) {
+    rc = graph_predictions(g, model, &pred);
+    if (rc) return rc;
+  }
   PagedGraph* pg = nullptr;
   if (paged) {
     rc = ensure_paged(g, opts->cache_bytes, opts->package_bytes ? opts->package_bytes : (2u << 20),
@@ -1881,6 +1972,12 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
           : (uint32_t)std::max<uint64_t>(
                 1, std::min<uint64_t>(64, npos * npat / ((uint64_t)waves * 8)));
   const uint64_t nchunk = (npos + chunk - 1) / chunk;
+  // learned order applies to full-graph, unsampled runs
+  const bool lorder = pred && (opts->learned & PM_LEARN_ORDER) && sample == 1 && lo == 0 && hi == h.m;
+  if (lorder) {
+    rc = learned_order(g, model, *pred);
+    if (rc) return rc;
+  }
   for (KPlan& kp : kps) {
     kp.split = opts->split > 0 ? (uint32_t)opts->split : 8192u;
     kp.hub_factor = opts->hub_factor > 0 ? (uint32_t)opts->hub_factor : 8u;
@@ -1992,6 +2089,33 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   L.arena_stride = stride;
   L.out = g_ws.out;
   L.tasks = (any_split && opts->split >= 0) ? g_ws.tasks : nullptr;
+  L.root_order = lorder ? g->d_root_order : nullptr;
+  L.root_start = lorder ? g->d_root_start : nullptr;
+  unsigned long long* d_root_work = nullptr;
+  if (opts->root_work && h.n) {
+    HIP_OK(hipMallocAsync((void**)&d_root_work, sizeof(unsigned long long) * h.n, st));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, st, d_root_work, h.n, 0ull);
+  }
+  // learned paging: a package's score is the mean predicted use of its vertices
+  std::vector<float> pscore;
+  std::vector<uint32_t> by_score;  // packages, highest score first
+  if (paged && pred && (opts->learned & PM_LEARN_PAGING)) {
+    const PackageLayout& PLo = pg->layout;
+    pscore.assign(PLo.count(), 0.0f);
+    for (uint32_t q = 0; q < PLo.count(); q++) {
+      const uint32_t v0 = PLo.first_vertex[q];
+      const uint32_t v1 = q + 1 < PLo.count() ? PLo.first_vertex[q + 1] : (uint32_t)h.n;
+      double sum = 0;
+      for (uint32_t v = v0; v < v1; v++) sum += (*pred)[2 * v + 1];
+      pscore[q] = v1 > v0 ? (float)(sum / (v1 - v0)) : 0.0f;
+    }
+    by_score.resize(PLo.count());
+    std::iota(by_score.begin(), by_score.end(), 0u);
+    std::stable_sort(by_score.begin(), by_score.end(),
+                     [&](uint32_t a, uint32_t b) { return pscore[a] > pscore[b]; });
+  }
+  std::vector<uint64_t> use_acc;  // trace: rounds each package was used in
+  if (paged && opts->package_use) use_acc.assign(pg->layout.count(), 0);
   L.task_cap = g_ws.task_cap;
   L.rout_cap = rcap;
   const int reps = opts->repeats > 0 ? opts->repeats : 1;
@@ -2006,6 +2130,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.out, 8, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, L.pcount, 2ull * npat, 0ull);
     L.per_edge = (r == 0) ? d_per_edge : nullptr;
+    L.root_work = (r == 0) ? d_root_work : nullptr;
     if (L.per_edge)
       hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, st, d_per_edge, hi - lo, 0ull);
     const auto t0 = std::chrono::steady_clock::now();
@@ -2016,11 +2141,18 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       // empty cache, then the reference's initial preload (buffer_init_on_gpu,
       // src/buffer.cu:264-415): packages in id order until the cache is full
       pg->policy.reset((uint32_t)pg->nslots, opts->policy);
+      if (!pscore.empty()) pg->policy.set_scores(pscore);
       pg->pkg_slot.assign(np, -1);
       std::vector<uint8_t> pinned(pg->nslots, 0);
       std::vector<uint32_t> pre;
       uint64_t used_slots = 0;
-      for (uint32_t p = 0; p < np && used_slots + pg->layout.nslots[p] <= pg->nslots; p++) {
+      // learned prefetch preloads the packages of highest predicted use
+      for (uint32_t i = 0; i < np; i++) {
+        const uint32_t p = (opts->prefetch && !by_score.empty()) ? by_score[i] : i;
+        if (used_slots + pg->layout.nslots[p] > pg->nslots) {
+          if (by_score.empty() || !opts->prefetch) break;
+          continue;
+        }
         pre.push_back(p);
         used_slots += pg->layout.nslots[p];
       }
@@ -2110,9 +2242,20 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
         const bool edges_left = cur8[0] < span;
         const uint64_t tasks_done = std::min(cur8[1], cur8[2]);
         const bool tasks_left = L.tasks && tasks_done < cur8[2];
-        if (pending.empty() && !edges_left && !tasks_left) break;
         std::vector<uint8_t> used(pg->nslots);
         HIP_OK(hipMemcpy(used.data(), pg->d_used, pg->nslots, hipMemcpyDeviceToHost));
+        if (!use_acc.empty()) {
+          // trace: each package used this round counts once
+          std::vector<uint8_t> seen(np, 0);
+          for (uint64_t sl = 0; sl < pg->nslots; sl++) {
+            const int32_t own = pg->policy.owner((uint32_t)sl);
+            if (used[sl] && own >= 0 && !seen[own]) {
+              seen[own] = 1;
+              use_acc[own]++;
+            }
+          }
+        }
+        if (pending.empty() && !edges_left && !tasks_left) break;
         pg->policy.touched(used.data(), (uint32_t)pg->nslots);
         // admission: oldest pieces first while their needs fit the cache
         auto needs = [&](const Resume& d, uint32_t* need) {
@@ -2157,6 +2300,29 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
         rc = load_packages(pg, want, pinned, g_ws.copy_stream, &b);
         if (rc) return rc;
         link_bytes += b;
+        if (opts->prefetch && !by_score.empty()) {
+          // learned prefetch: free slots take the packages of highest
+          // predicted use that are not resident (nothing is evicted for it)
+          uint64_t free_slots = 0;
+          std::vector<uint8_t> keep(pg->nslots, 1);
+          for (uint64_t sl = 0; sl < pg->nslots; sl++)
+            if (pg->policy.owner((uint32_t)sl) < 0) {
+              free_slots++;
+              keep[sl] = 0;
+            }
+          std::vector<uint32_t> pre;
+          for (uint32_t q : by_score) {
+            if (free_slots == 0) break;
+            if (pg->pkg_slot[q] >= 0 || PL.nslots[q] > free_slots) continue;
+            pre.push_back(q);
+            free_slots -= PL.nslots[q];
+          }
+          if (!pre.empty()) {
+            rc = load_packages(pg, pre, keep, g_ws.copy_stream, &b);
+            if (rc) return rc;
+            link_bytes += b;
+          }
+        }
         // resume every piece whose needs are now resident (the admitted ones
         // and any other that finds its packages in the cache)
         std::vector<Resume> admitted, rest;
@@ -2220,7 +2386,19 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
                           hipMemcpyDeviceToHost, st));
     HIP_OK(hipFreeAsync(d_per_edge, st));
   }
+  if (d_root_work) {
+    HIP_OK(hipMemcpyAsync(opts->root_work, d_root_work, sizeof(unsigned long long) * h.n,
+                          hipMemcpyDeviceToHost, st));
+    HIP_OK(hipFreeAsync(d_root_work, st));
+  }
   HIP_OK(hipStreamSynchronize(st));
+  if (opts->package_use) {
+    // per vertex: the rounds in which its package was used (0 when resident)
+    for (uint64_t v = 0; v < h.n; v++)
+      opts->package_use[v] =
+          use_acc.empty() ? 0u
+                          : (uint32_t)std::min<uint64_t>(use_acc[pg->layout.pkg_of[v]], 0xFFFFFFFFu);
+  }
   if (!paged && cur[4] != 0) {
     set_error("internal: a resident-row run deferred work");
     return PM_ERR_DEVICE;

@@ -53,6 +53,12 @@ struct OrderStats {
 };
 }  // namespace pm
 
+struct pm_model {
+  std::vector<float> w, adam_m, adam_v;  // PM_MODEL_PARAMS each
+  int64_t step = 0;
+  uint64_t version = 0;  // unique per parameter state (process-wide counter)
+};
+
 struct pm_graph {
   pm::HostGraph h;
   pm::DeviceGraph d;
@@ -60,6 +66,15 @@ struct pm_graph {
   int device = 0;
   bool stats_ok = false;  // cleared when labels change
   pm::OrderStats stats;
+  // model predictions per vertex (n x 2), for pred_model at pred_version
+  const pm_model* pred_model = nullptr;
+  uint64_t pred_version = 0;
+  std::vector<float> pred;
+  // learned job order: roots by predicted work, and the first job position
+  // of each (device), for order_version
+  uint32_t* d_root_order = nullptr;
+  uint64_t* d_root_start = nullptr;
+  uint64_t order_version = 0;
 };
 
 namespace pm {
@@ -155,6 +170,7 @@ void build_label_grouped(const HostGraph& g, std::vector<uint32_t>* lcol,
 
 // device (match.hip)
 int device_upload(pm_graph* g);
+int device_upload_rows(pm_graph* g, bool with_rows);
 void device_release(pm_graph* g);
 int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res);
 // One launch over several patterns; res[i] gets pattern i's count and jobs
@@ -163,5 +179,15 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
                        pm_result* res);
 int autotune(pm_graph* g, pm_pattern* const* p, int n, const pm_opts* base,
              const pm_tune_opts* t, pm_opts* best, pm_tune_report* rep);
+// learned parts (learn.hip)
+uint64_t next_model_version();
+void model_init(pm_model* m, uint64_t seed);
+int model_predict_host(const pm_model* m, const float* X, uint64_t n, float* Y);
+int model_grad(const pm_model* m, const float* X, const float* Y, const float* Wt, uint64_t n,
+               float* grad, double* loss);
+int model_train(pm_model* m, const float* X, const float* Y, const float* Wt, uint64_t n,
+                const pm_train_opts* o, double* final_loss);
+int graph_features_host(pm_graph* g, float* out);
+int graph_predictions(pm_graph* g, const pm_model* m, const std::vector<float>** out);
 
 }  // namespace pm

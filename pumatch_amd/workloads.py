@@ -51,3 +51,27 @@ def batch_patterns(count: int = 64, seed: int = 6, nlabels: int = 8, kmin: int =
         labels = [rng.randrange(nlabels) for _ in range(k)]
         out.append(pattern_g_text(adj, labels))
     return out
+
+
+def permuted(g, seed: int, device: bool = True):
+    """Copy of graph g with its vertex ids relabelled by a seeded random
+    permutation (labels move with their vertices).  Generated graphs number
+    vertices by expected degree; real inputs do not."""
+    import numpy as np
+
+    from . import Graph
+
+    rp, ci, lb = g.csr()
+    n = rp.size - 1
+    perm = np.random.default_rng(seed).permutation(n).astype(np.int64)
+    src = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    a, b = perm[src], perm[ci.astype(np.int64)]
+    order = np.lexsort((b, a))
+    rp2 = np.zeros(n + 1, np.uint64)
+    np.add.at(rp2, a + 1, 1)
+    rp2 = np.cumsum(rp2).astype(np.uint64)
+    labels = None
+    if lb is not None and lb.size:
+        labels = np.zeros(n, np.uint32)
+        labels[perm] = lb
+    return Graph.from_csr(rp2, b[order].astype(np.uint32), labels=labels, device=device)
