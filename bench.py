@@ -1,20 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark: ms per 7-vertex pattern on one MI355X, exact count.
 
-Workload (one "step" = one full match of the pattern over the graph):
+Main line (one "step" = one full match of the pattern over the graph):
   RMAT scale 16 (65,536 vertices), edge factor 16, (a,b,c,d) =
   (0.57,0.19,0.19,0.05), seed 1; symmetrized, deduplicated, self-loops
   removed, adjacency sorted (u32 ids, u64 offsets); 4 uniform labels
   (seed 2); labeled pattern K4{0,1,2,3} + path 3-4-5-6, every vertex label 1.
-The graph is resident in HBM before the timed region.  Each step is
-synchronous (the library waits on its stream before returning), so the
-wall-clock bracket around K steps covers exactly K complete matches.
+The graph is resident in HBM before the timed region.  Launch granularities
+come from the autotuner (pm_autotune, sampled launches before timing); the
+untuned time is reported beside.  Each step is synchronous (the library
+waits on its stream before returning), so the wall-clock bracket around K
+steps covers exactly K complete matches.
 
 The JSON line carries the roofline of the dominant kernel (algorithmic
-bytes per launch / its mean duration from hipEvents on the launch stream)
-and a CPU baseline: the oracle (oracle/oracle.c, OpenMP) timed on a bounded
-sample of the same jobs and scaled to the whole job set by the ratio of
-matches (the oracle's work grows with the number of matches it enumerates).
+bytes per launch / its mean duration from hipEvents on the launch stream;
+HBM traffic from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised in
+profiles/) and a CPU baseline: the oracle (oracle/oracle.c, OpenMP) timed on
+a bounded sample of the same jobs and scaled to the whole job set by the
+ratio of matches.
+
+Two more objects compare plain and learned+batched schedules:
+  "batched": 64 random connected labeled patterns (3-7 vertices, density
+    0.3-0.8, seed 6) on RMAT scale 14 with 8 labels (seed 5): plain =
+    one pm_match per pattern in file order with default options; batched =
+    one pm_match_batch launch with autotuned orders and granularities.
+  "learned": out-of-core runs on a Chung-Lu graph (50k vertices, exponent
+    2.2, max degree 500, mean 20, ids permuted) with 40% of its rows in
+    the HBM cache: plain LRU vs the MLP trained on traces of seeds 7-9 and
+    evaluated on seed 10 (learned prefetch + eviction).
 """
 import argparse
 import json
@@ -27,10 +40,12 @@ sys.path.insert(0, ROOT)
 
 C1_EDGES = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (3, 4), (4, 5), (5, 6)]
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_match_edges_rmat16_c1.json")
 
 
-def pattern_text(k, edges, label=1):
-    lines = ["# t 0"] + [f"v {i} {label}" for i in range(k)]
+def pattern_text(k, edges, labels=None):
+    labels = labels or [1] * k
+    lines = ["# t 0"] + [f"v {i} {labels[i]}" for i in range(k)]
     for a, b in edges:
         lines += [f"e {a} {b} 1", f"e {b} {a} 1"]
     return "\n".join(lines) + "\n"
@@ -79,6 +94,125 @@ def cpu_baseline(g, text, per_edge, total_count, seconds, threads):
     }
 
 
+def batched_section(pm):
+    from pumatch_amd.workloads import batch_patterns
+
+    g = pm.Graph.rmat(14, 16, seed=5)
+    g.set_uniform_labels(8, 5)
+    texts = batch_patterns(64, seed=6, nlabels=8)
+    pats = [pm.Pattern.parse(t, True) for t in texts]
+    n7 = sum(1 for t in texts if t.count("\nv ") == 7)
+    plain_ms, plain = 0.0, []
+    for p in pats:
+        r = pm.match(g, p, order="file")
+        plain.append(r.count)
+        plain_ms += r.kernel_ms
+    t0 = time.perf_counter()
+    opts, rep = pm.autotune(g, pats, sample=4)
+    tune_s = time.perf_counter() - t0
+    rs = pm.match_batch(g, pats, repeats=3, **opts)
+    assert [r.count for r in rs] == plain, "batched counts differ from single-pattern runs"
+    batch_ms = rs[0].min_ms
+    return {
+        "workload": ("rmat14-ef16-labeled8: RMAT scale 14, edge factor 16, seed 5, 8 labels; "
+                     "64 random connected labeled patterns, 3-7 vertices, density 0.3-0.8, seed 6"),
+        "patterns": len(pats),
+        "patterns_7_vertices": n7,
+        "plain_sequential_ms": round(plain_ms, 3),
+        "plain_ms_per_pattern": round(plain_ms / len(pats), 4),
+        "batched_tuned_ms": round(batch_ms, 3),
+        "batched_ms_per_pattern": round(batch_ms / len(pats), 4),
+        "speedup": round(plain_ms / batch_ms, 3),
+        "counts_equal": True,
+        "autotune_s": round(tune_s, 2),
+        "autotune": {"chunk": opts["chunk"], "split": opts["split"],
+                     "hub_factor": opts["hub_factor"], "reordered_patterns": rep["reordered"],
+                     "sample": rep["sample"]},
+    }
+
+
+def learned_section(pm):
+    import numpy as np
+
+    from pumatch_amd.workloads import permuted
+
+    N, PKG = 50000, 16384
+
+    def graph(seed):
+        g0 = pm.Graph.chung_lu(N, 2.2, N // 100, 20, seed=seed, device=False)
+        g = permuted(g0, seed + 100)
+        g.set_uniform_labels(8, seed)
+        return g
+
+    def text(k, edges):
+        return pattern_text(k, edges)
+
+    pats = [("triangle", pm.Pattern.parse(text(3, [(0, 1), (1, 2), (0, 2)]), False)),
+            ("k4", pm.Pattern.parse(text(4, [(a, b) for a in range(4) for b in range(a)]), False))]
+    Xs, Ys = [], []
+    for seed in (7, 8, 9):
+        g = graph(seed)
+        cache = int(g.m * 4 * 0.4) // PKG * PKG
+        rw, pu = np.zeros(g.n), np.zeros(g.n)
+        for _, p in pats:
+            r = pm.match(g, p, traces=True, cache_bytes=cache, package_bytes=PKG)
+            rw += r.root_work
+            pu += r.package_use
+        Xs.append(pm.features(g))
+        Ys.append(np.stack([np.log2(1 + rw), np.log2(1 + pu)], 1))
+    X = np.concatenate(Xs)
+    Y = np.concatenate(Ys).astype(np.float32)
+    m = pm.Model(1)
+    t0 = time.perf_counter()
+    loss = m.train(X, Y, epochs=30, lr=3e-3)
+    train_s = time.perf_counter() - t0
+    g = graph(10)
+    cache = int(g.m * 4 * 0.4) // PKG * PKG
+    out = {"workload": (f"chunglu50k: Chung-Lu n={N}, exponent 2.2, max degree {N // 100}, mean 20, "
+                        f"ids permuted; {cache} B HBM cache of {PKG} B packages (40% of rows); "
+                        "unlabeled triangle and K4"),
+           "train": f"traces of plain LRU runs on seeds 7-9 ({X.shape[0]} roots), "
+                    f"{train_s:.2f} s on the GPU, final loss {loss:.3f}",
+           "eval_seed": 10, "patterns": {}}
+    tp = tl = 0.0
+    for name, p in pats:
+        a = pm.match(g, p, cache_bytes=cache, package_bytes=PKG)
+        b = pm.match(g, p, cache_bytes=cache, package_bytes=PKG, policy="learned", prefetch=True,
+                     model=m)
+        assert a.count == b.count
+        out["patterns"][name] = {"plain_ms": round(a.kernel_ms, 2), "learned_ms": round(b.kernel_ms, 2),
+                                 "plain_rounds": a.rounds, "learned_rounds": b.rounds,
+                                 "plain_link_MB": round(a.link_bytes / 1e6, 2),
+                                 "learned_link_MB": round(b.link_bytes / 1e6, 2),
+                                 "plain_link_GBps": round(a.link_bytes / max(a.link_ms, 1e-9) / 1e6, 2),
+                                 "count": a.count}
+        tp += a.kernel_ms
+        tl += b.kernel_ms
+    out["plain_ms"] = round(tp, 2)
+    out["learned_ms"] = round(tl, 2)
+    out["speedup"] = round(tp / tl, 3)
+    return out
+
+
+def copy_calibration():
+    """A plain device-to-device copy of known size (the runtime's blit
+    kernel), for scaling the FETCH_SIZE and WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = 1 << 30  # bytes
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,11 +220,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned objects")
+    ap.add_argument("--no-tune", action="store_true", help="time default launch options")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the matcher shards by job ranges across GPUs "
               "(see DESIGN.md), but only --gpus 1 is run here", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import numpy as np
 
@@ -104,19 +244,27 @@ def main():
     print(f"[bench] graph n={g.n} m={g.m} built in {time.perf_counter() - t0:.1f}s", flush=True)
 
     first = pm.match(g, p, per_edge=True)  # warm-up with per-edge counts for the baseline
+    opts, tune = {}, None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        opts, tune = pm.autotune(g, [p], sample=1)
+        tune["seconds"] = round(time.perf_counter() - t0, 2)
     for _ in range(max(0, args.warmup - 1)):
-        pm.match(g, p)
-    print(f"[bench] warm-up done count={first.count}", flush=True)
+        pm.match(g, p, **opts)
+    print(f"[bench] warm-up done count={first.count} opts={opts}", flush=True)
 
     results = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        results.append(pm.match(g, p))
+        results.append(pm.match(g, p, **opts))
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     counts = {r.count for r in results} | {first.count}
     assert len(counts) == 1, f"count changed between steps: {counts}"
     count = first.count
+    untuned = [pm.match(g, p) for _ in range(3)]
+    assert all(r.count == count for r in untuned)
+    untuned_ms = float(np.mean([r.kernel_ms for r in untuned]))
 
     # roofline of the dominant kernel phase
     names = ["k_fold", "k_match_edges", "k_match_tasks"]
@@ -125,6 +273,10 @@ def main():
     dom = int(np.argmax(ph_ms))
     achieved = ph_bytes[dom] / (ph_ms[dom] * 1e-3) / 1e9 if ph_ms[dom] > 0 else 0.0
     kernel_ms = float(np.mean([r.kernel_ms for r in results]))
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and dom == 1:
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {
         "bound": "hbm",
         "kernel": names[dom],
@@ -132,7 +284,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": traffic,
         "kernel_ms": round(float(ph_ms[dom]), 4),
         "alg_bytes": int(ph_bytes[dom]),
     }
@@ -143,8 +295,15 @@ def main():
         print(f"[bench] cpu baseline ({threads} threads, ~{args.cpu_seconds}s)", flush=True)
         cpu = cpu_baseline(g, text, first.per_edge, count, args.cpu_seconds, threads)
 
+    extras = {}
+    if not args.no_extras:
+        print("[bench] batched: 64 patterns, plain sequential vs tuned batch", flush=True)
+        extras["batched"] = batched_section(pm)
+        print("[bench] learned paging: train on seeds 7-9, evaluate on seed 10", flush=True)
+        extras["learned"] = learned_section(pm)
+
     out = {
-        "metric": "ms per 7-vertex pattern (plain GPU), exact count, HBM GB/s",
+        "metric": "ms per 7-vertex pattern (learned+batched vs plain GPU), exact count, HBM GB/s",
         "value": round(ms_per_step, 4),
         "unit": "ms",
         "n_gpus": 1,
@@ -167,12 +326,16 @@ def main():
         "count": count,
         "jobs": first.jobs,
         "kernel_ms": round(kernel_ms, 4),
+        "untuned_kernel_ms": round(untuned_ms, 4),
+        "tuned_options": opts,
+        "autotune": tune,
         "alg_GBps": round(results[-1].alg_bytes / (kernel_ms * 1e-3) / 1e9, 2),
         "phase_ms": [round(float(x), 4) for x in ph_ms],
         "tasks": results[-1].tasks,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 
