@@ -40,7 +40,7 @@ sys.path.insert(0, ROOT)
 
 C1_EDGES = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (3, 4), (4, 5), (5, 6)]
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_match_edges_rmat16_c1.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_match_edges_rmat16_c1.txt")
 
 
 def pattern_text(k, edges, labels=None):
@@ -275,8 +275,12 @@ def main():
     kernel_ms = float(np.mean([r.kernel_ms for r in results]))
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and dom == 1:
+        # measured offline: rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by
+        # the same counters on a copy of known size (the file gives the commands)
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    traffic = int(line.split(":")[1])
     roofline = {
         "bound": "hbm",
         "kernel": names[dom],
