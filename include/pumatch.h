@@ -106,6 +106,8 @@ typedef struct {
   int32_t rounds;       /* out-of-core: matching rounds (1 when resident) */
   int32_t packages;     /* out-of-core: packages in the graph */
   double link_ms;       /* out-of-core: host->HBM copy time */
+  double round_kernel_ms; /* out-of-core: kernel time of rounds 2.. (the rest of
+                           kernel_ms minus link_ms is host-side round work) */
 } pm_result;
 
 typedef struct {

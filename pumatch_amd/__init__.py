@@ -90,6 +90,7 @@ class Result(C.Structure):
         ("rounds", C.c_int32),
         ("packages", C.c_int32),
         ("link_ms", C.c_double),
+        ("round_kernel_ms", C.c_double),
     ]
 
 
@@ -338,6 +339,7 @@ class MatchResult:
     packages: int = 0
     link_ms: float = 0.0
     per_edge: np.ndarray | None = None
+    round_kernel_ms: float = 0.0
     root_work: np.ndarray | None = None
     package_use: np.ndarray | None = None
 
@@ -447,7 +449,7 @@ def _result(r: Result, pe=None) -> MatchResult:
                        int(r.set_ops), int(r.link_bytes), int(r.tasks), int(r.waves), int(r.chunk),
                        tuple(r.phase_ms), tuple(int(x) for x in r.phase_alg_bytes),
                        int(r.misses), int(r.deferred), int(r.rounds), int(r.packages),
-                       float(r.link_ms), pe)
+                       float(r.link_ms), pe, float(r.round_kernel_ms))
 
 
 def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int = 1,

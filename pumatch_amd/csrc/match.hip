@@ -70,6 +70,16 @@ constexpr int kWave = 64;
 constexpr uint32_t kInf = 0xFFFFFFFFu;
 constexpr int kHashBits = 9;
 constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
+// build-time tunables: fused-walk entries in flight per lane, and the
+// minimum waves per SIMD the resident kernels are compiled for (the LDS
+// state allows 5-6)
+#ifndef PM_FUSED_WALK
+#define PM_FUSED_WALK 4
+#endif
+#ifndef PM_RESIDENT_WAVES
+#define PM_RESIDENT_WAVES 5
+#endif
+constexpr int kWalk = PM_FUSED_WALK;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kMaxDepth = 6;
 
@@ -817,30 +827,45 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
     __syncthreads();
     const uint32_t total = uni(ws.own_off[kWave]);
     ct.alg += 4ull * total + 16ull * kWave;
-    for (uint32_t t = lane; t < total; t += kWave) {
-      // owner: last o with own_off[o] <= t
-      uint32_t lo = 0, hi = kWave;
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ws.own_off[mid] <= t) lo = mid; else hi = mid;
-      }
-      const uint32_t y = ((const uint32_t*)ws.own_ptr[lo])[t - ws.own_off[lo]];
-      if (y >= ws.own_ub[lo]) continue;
-      bool hit = true;
-      if (isect) {
-        if (use_hash) {
-          uint32_t h = hslot(y);
-          uint32_t e;
-          while ((e = ws.hash[h]) != kEmpty && e != y) h = (h + 1) & (kHashCap - 1);
-          hit = e == y;
-        } else {
-          hit = member(Pset, nP, y);
+    // kWalk entries per lane per step: their row loads, then their weight
+    // gathers, are issued together so the latencies overlap
+    for (uint32_t t0 = lane; t0 < total; t0 += kWalk * kWave) {
+      uint32_t yv[kWalk], ov[kWalk];
+#pragma unroll
+      for (int q = 0; q < kWalk; q++) {
+        const uint32_t t = t0 + q * kWave;
+        // owner: last o with own_off[o] <= t
+        uint32_t lo = 0, hi = kWave;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (ws.own_off[mid] <= t) lo = mid; else hi = mid;
         }
+        ov[q] = lo;
+        yv[q] = t < total ? ((const uint32_t*)ws.own_ptr[lo])[t - ws.own_off[lo]] : kInf;
       }
-      if (hit && check2) hit = g.lab[y] == P.lab[u2];
-      if (hit) {
-        acc += ws.own_w[lo] * (w2 ? w2[y] : 1ull);
-        if (pair) acc += ws.own_b[lo] * (w1 ? w1[y] : 1ull);
+      unsigned long long wv2[kWalk], wv1[kWalk];
+#pragma unroll
+      for (int q = 0; q < kWalk; q++) {
+        const uint32_t y = yv[q];
+        bool hit = y < ws.own_ub[ov[q]];  // also false past the end (y = kInf)
+        if (hit && isect) {
+          if (use_hash) {
+            uint32_t h = hslot(y);
+            uint32_t e;
+            while ((e = ws.hash[h]) != kEmpty && e != y) h = (h + 1) & (kHashCap - 1);
+            hit = e == y;
+          } else {
+            hit = member(Pset, nP, y);
+          }
+        }
+        if (hit && check2) hit = g.lab[y] == P.lab[u2];
+        wv2[q] = hit ? (w2 ? w2[y] : 1ull) : 0ull;
+        wv1[q] = (hit && pair) ? (w1 ? w1[y] : 1ull) : 0ull;
+      }
+#pragma unroll
+      for (int q = 0; q < kWalk; q++) {
+        acc += ws.own_w[ov[q]] * wv2[q];
+        if (pair) acc += ws.own_b[ov[q]] * wv1[q];
       }
     }
     __syncthreads();
@@ -1172,7 +1197,7 @@ __device__ __forceinline__ DeferOut defer_out(const Launch& L) {
 }
 
 template <bool PG>
-__global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
+__global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_edges(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
   const uint32_t lane = threadIdx.x;
@@ -1271,7 +1296,7 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_edges(Launch L) {
 
 // Range tasks (phase 2).
 template <bool PG>
-__global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_tasks(Launch L) {
+__global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_tasks(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
   const uint32_t lane = threadIdx.x;
@@ -1311,7 +1336,7 @@ __global__ __launch_bounds__(64, PG ? 4 : 6) void k_match_tasks(Launch L) {
 
 // Resume round: the admitted deferred pieces (a depth-0 piece may split).
 template <bool PG>
-__global__ __launch_bounds__(64, PG ? 4 : 6) void k_resume(Launch L) {
+__global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_resume(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
   const uint32_t lane = threadIdx.x;
@@ -1427,6 +1452,36 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     p[i] = v;
 }
 
+// One round's package loads in one launch: wave w copies job w's rows from
+// pinned host memory (read by the GPU over the link) into its cache slots.
+struct CopyJob {
+  uint64_t src;  // word offset in the host rows
+  uint64_t dst;  // word offset in the cache
+  uint64_t words;
+};
+
+__global__ __launch_bounds__(256) void k_copy_packages(const uint32_t* __restrict__ host,
+                                                       uint32_t* __restrict__ cache,
+                                                       const CopyJob* __restrict__ jobs,
+                                                       uint32_t njobs) {
+  const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (uint32_t j = blockIdx.x * (blockDim.x / kWave) + wave; j < njobs;
+       j += gridDim.x * (blockDim.x / kWave)) {
+    const CopyJob c = jobs[j];
+    const uint32_t* src = host + c.src;
+    uint32_t* dst = cache + c.dst;
+    uint64_t i = lane;
+    for (; i + 3 * kWave < c.words; i += 4 * kWave) {
+      const uint32_t a = src[i], b = src[i + kWave], d = src[i + 2 * kWave], e = src[i + 3 * kWave];
+      dst[i] = a;
+      dst[i + kWave] = b;
+      dst[i + 2 * kWave] = d;
+      dst[i + 3 * kWave] = e;
+    }
+    for (; i < c.words; i += kWave) dst[i] = src[i];
+  }
+}
+
 #define HIP_OK(call)                                                \
   do {                                                              \
     hipError_t _e = (call);                                         \
@@ -1455,6 +1510,10 @@ struct Workspace {
   size_t table_bytes = 0;
   void* resume[2] = {nullptr, nullptr};
   size_t resume_bytes[2] = {0, 0};
+  Resume* h_fresh = nullptr;  // pinned staging: pieces device -> host
+  size_t h_fresh_cap = 0;
+  Resume* h_admit = nullptr;  // pinned staging: admitted pieces host -> device
+  size_t h_admit_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, eve = nullptr, evc = nullptr;
   int cus = 0;
   int resident = 0;
@@ -1478,6 +1537,17 @@ int ensure_workspace(int device) {
   g_ws.task_cap = 1ull << 22;  // 4M range tasks (96 MiB)
   HIP_OK(hipMalloc(&g_ws.tasks, sizeof(Task) * g_ws.task_cap));
   g_ws.device = device;
+  return PM_OK;
+}
+
+int grow_pinned(Resume** p, size_t* have, size_t need) {
+  if (need <= *have) return PM_OK;
+  const size_t n = std::max(need, *have * 2);
+  if (*p) HIP_OK(hipHostFree(*p));
+  *p = nullptr;
+  *have = 0;
+  HIP_OK(hipHostMalloc((void**)p, sizeof(Resume) * n, hipHostMallocDefault));
+  *have = n;
   return PM_OK;
 }
 
@@ -1697,6 +1767,8 @@ struct PagedGraph {
   unsigned long long* d_miss_count = nullptr;
   uint32_t miss_cap = 0;
   uint8_t* d_used = nullptr;
+  CopyJob* d_jobs = nullptr;       // package copies of one call (<= nslots)
+  std::vector<CopyJob> jobs;       // host staging of the same
   std::vector<int32_t> pkg_slot;  // host mirror
   CachePolicy policy;
 };
@@ -1724,6 +1796,7 @@ void device_release_paging(pm_graph* g) {
     (void)hipFree(pg->d_miss_list);
     (void)hipFree(pg->d_miss_count);
     (void)hipFree(pg->d_used);
+    (void)hipFree(pg->d_jobs);
     g_paged.erase(g_paged.begin() + i);
     return;
   }
@@ -1754,6 +1827,7 @@ int ensure_paged(pm_graph* g, uint64_t cache_bytes, uint32_t package_bytes, Page
         (void)hipFree(q->d_miss_list);
         (void)hipFree(q->d_miss_count);
         (void)hipFree(q->d_used);
+        (void)hipFree(q->d_jobs);
         g_paged.erase(g_paged.begin() + i);
         break;
       }
@@ -1785,6 +1859,7 @@ int ensure_paged(pm_graph* g, uint64_t cache_bytes, uint32_t package_bytes, Page
   HIP_OK(hipMalloc(&pg->d_miss_list, sizeof(uint32_t) * pg->miss_cap));
   HIP_OK(hipMalloc(&pg->d_miss_count, sizeof(unsigned long long)));
   HIP_OK(hipMalloc(&pg->d_used, nslots));
+  HIP_OK(hipMalloc(&pg->d_jobs, sizeof(CopyJob) * std::max<uint64_t>(nslots, 1)));
   if (h.n)
     HIP_OK(hipMemcpy(pg->d_pkg_of, pg->layout.pkg_of.data(), sizeof(uint32_t) * h.n,
                      hipMemcpyHostToDevice));
@@ -1819,12 +1894,24 @@ int load_packages(PagedGraph* pg, const std::vector<uint32_t>& want, std::vector
     pg->policy.assign((uint32_t)s, width, (int32_t)p);
     for (uint32_t w = 0; w < width; w++) pinned[s + w] = 1;
     pg->pkg_slot[p] = (int32_t)s;
-    const uint64_t nbytes = L.words[p] * 4;
-    if (nbytes)
-      HIP_OK(hipMemcpyAsync(pg->d_cache + (uint64_t)s * pg->slot_words,
-                            pg->host_rows + L.edge_base[p], nbytes, hipMemcpyHostToDevice, cs));
-    *bytes += nbytes;
+    if (L.words[p]) pg->jobs.push_back({L.edge_base[p], (uint64_t)s * pg->slot_words, L.words[p]});
+    *bytes += L.words[p] * 4;
   }
+  return PM_OK;
+}
+
+// Issue the package copies gathered by load_packages (one launch on cs).
+int flush_packages(PagedGraph* pg, hipStream_t cs) {
+  if (pg->jobs.empty()) return PM_OK;
+  HIP_OK(hipMemcpyAsync(pg->d_jobs, pg->jobs.data(), sizeof(CopyJob) * pg->jobs.size(),
+                        hipMemcpyHostToDevice, cs));
+  const uint32_t nj = (uint32_t)pg->jobs.size();
+  hipLaunchKernelGGL(k_copy_packages, dim3(std::min<uint32_t>((nj + 3) / 4, 4096)), dim3(256), 0, cs,
+                     pg->host_rows_dev, pg->d_cache, pg->d_jobs, nj);
+  HIP_OK(hipGetLastError());
+  // the staging vector is reused after the stream has drained
+  HIP_OK(hipStreamSynchronize(cs));
+  pg->jobs.clear();
   return PM_OK;
 }
 
@@ -2119,7 +2206,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   L.task_cap = g_ws.task_cap;
   L.rout_cap = rcap;
   const int reps = opts->repeats > 0 ? opts->repeats : 1;
-  double sum_ms = 0, min_ms = 1e30, phase[3] = {0, 0, 0}, link_ms = 0;
+  double sum_ms = 0, min_ms = 1e30, phase[3] = {0, 0, 0}, link_ms = 0, round_kernel_ms = 0;
   uint64_t link_bytes = 0, misses = 0, deferred = 0;
   int rounds = 1;
   const int fold_blocks = (int)std::max<uint64_t>(
@@ -2136,6 +2223,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     const auto t0 = std::chrono::steady_clock::now();
     link_bytes = misses = deferred = 0;
     link_ms = 0;
+    round_kernel_ms = 0;
     rounds = 1;
     if (paged) {
       // empty cache, then the reference's initial preload (buffer_init_on_gpu,
@@ -2159,6 +2247,8 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       HIP_OK(hipEventRecord(g_ws.evc, g_ws.copy_stream));
       uint64_t b = 0;
       rc = load_packages(pg, pre, pinned, g_ws.copy_stream, &b);
+      if (rc) return rc;
+      rc = flush_packages(pg, g_ws.copy_stream);
       if (rc) return rc;
       link_bytes += b;
       HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
@@ -2217,7 +2307,14 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       const PackageLayout& PL = pg->layout;
       const uint64_t span = (uint64_t)npat * nchunk;  // phase-1 units
       const size_t admit_cap = 1u << 18;  // pieces resumed per round
-      std::vector<Resume> pending, fresh;
+      // pending pieces, oldest first, live in pending[ph..]; a round looks
+      // at no more than 2 * admit_cap of them
+      // needs of a piece: the packages of its path rows and the package that
+      // stopped it, kNeed - 1 entries at most, computed once on arrival
+      constexpr int kNeed = kMaxDepth + 2;
+      std::vector<Resume> pending;
+      std::vector<uint32_t> pneed;  // kNeed per pending piece: count, packages
+      size_t ph = 0;
       for (;;) {
         unsigned long long cur8[8], mc = 0;
         HIP_OK(hipMemcpy(cur8, g_ws.cursor, sizeof(cur8), hipMemcpyDeviceToHost));
@@ -2234,10 +2331,21 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
         misses += mc;
         deferred += nd;
         if (nd) {
-          fresh.resize(nd);
-          HIP_OK(hipMemcpy(fresh.data(), g_ws.resume[1], sizeof(Resume) * nd,
+          rc = grow_pinned(&g_ws.h_fresh, &g_ws.h_fresh_cap, nd);
+          if (rc) return rc;
+          HIP_OK(hipMemcpy(g_ws.h_fresh, g_ws.resume[1], sizeof(Resume) * nd,
                            hipMemcpyDeviceToHost));
-          pending.insert(pending.end(), fresh.begin(), fresh.end());
+          pending.insert(pending.end(), g_ws.h_fresh, g_ws.h_fresh + nd);
+          const size_t base = pneed.size();
+          pneed.resize(base + nd * kNeed);
+          for (uint64_t i = 0; i < nd; i++) {
+            const Resume& d = g_ws.h_fresh[i];
+            uint32_t* q = &pneed[base + i * kNeed];
+            int c = 0;
+            for (int j = 0; j <= d.depth && j < kMaxDepth; j++) q[1 + c++] = PL.pkg_of[d.phi[j]];
+            q[1 + c++] = d.blk;
+            q[0] = (uint32_t)c;
+          }
         }
         const bool edges_left = cur8[0] < span;
         const uint64_t tasks_done = std::min(cur8[1], cur8[2]);
@@ -2255,23 +2363,23 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
             }
           }
         }
-        if (pending.empty() && !edges_left && !tasks_left) break;
+        if (ph == pending.size() && !edges_left && !tasks_left) break;
         pg->policy.touched(used.data(), (uint32_t)pg->nslots);
+        if (ph > 0 && ph * 2 >= pending.size()) {
+          pending.erase(pending.begin(), pending.begin() + (std::ptrdiff_t)ph);
+          pneed.erase(pneed.begin(), pneed.begin() + (std::ptrdiff_t)(ph * kNeed));
+          ph = 0;
+        }
+        const size_t wend = std::min(pending.size(), ph + admit_cap);
         // admission: oldest pieces first while their needs fit the cache
-        auto needs = [&](const Resume& d, uint32_t* need) {
-          int nneed = 0;
-          for (int j = 0; j <= d.depth && j < kMaxDepth; j++) need[nneed++] = PL.pkg_of[d.phi[j]];
-          need[nneed++] = d.blk;
-          return nneed;
-        };
         std::vector<uint8_t> in_set(np, 0);
         std::vector<uint32_t> want;
         uint64_t set_slots = 0;
         size_t nadmit = 0;
-        for (const Resume& d : pending) {
+        for (size_t i = ph; i < wend; i++) {
           if (nadmit >= admit_cap) break;
-          uint32_t need[kMaxDepth + 1];
-          const int nneed = needs(d, need);
+          const uint32_t* need = &pneed[i * kNeed + 1];
+          const int nneed = (int)pneed[i * kNeed];
           uint64_t extra = 0;
           for (int q = 0; q < nneed; q++) {
             bool dup = in_set[need[q]] != 0;
@@ -2287,7 +2395,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
           set_slots += extra;
           nadmit++;
         }
-        if (nadmit == 0 && !pending.empty() && !edges_left && !tasks_left) {
+        if (nadmit == 0 && ph < pending.size() && !edges_left && !tasks_left) {
           set_error("out-of-core: a deferred search needs more packages than the cache holds");
           return PM_ERR_OVERFLOW;
         }
@@ -2298,6 +2406,8 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
         const auto c0 = std::chrono::steady_clock::now();
         uint64_t b = 0;
         rc = load_packages(pg, want, pinned, g_ws.copy_stream, &b);
+        if (rc) return rc;
+        rc = flush_packages(pg, g_ws.copy_stream);
         if (rc) return rc;
         link_bytes += b;
         if (opts->prefetch && !by_score.empty()) {
@@ -2320,28 +2430,46 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
           if (!pre.empty()) {
             rc = load_packages(pg, pre, keep, g_ws.copy_stream, &b);
             if (rc) return rc;
+            rc = flush_packages(pg, g_ws.copy_stream);
+            if (rc) return rc;
             link_bytes += b;
           }
         }
-        // resume every piece whose needs are now resident (the admitted ones
-        // and any other that finds its packages in the cache)
-        std::vector<Resume> admitted, rest;
-        for (const Resume& d : pending) {
-          uint32_t need[kMaxDepth + 1];
-          const int nneed = needs(d, need);
-          bool here = admitted.size() < admit_cap;
+        // resume every piece of the window whose needs are now resident (the
+        // admitted ones and any other that finds its packages in the cache);
+        // the others stay pending in their order
+        std::vector<uint8_t> take(wend - ph, 0);
+        size_t ntake = 0;
+        for (size_t i = ph; i < wend && ntake < admit_cap; i++) {
+          const uint32_t* need = &pneed[i * kNeed + 1];
+          const int nneed = (int)pneed[i * kNeed];
+          bool here = true;
           for (int q = 0; q < nneed && here; q++) here = pg->pkg_slot[need[q]] >= 0;
-          (here ? admitted : rest).push_back(d);
+          if (here) {
+            take[i - ph] = 1;
+            ntake++;
+          }
         }
+        rc = grow_pinned(&g_ws.h_admit, &g_ws.h_admit_cap, std::max<size_t>(ntake, 1));
+        if (rc) return rc;
+        size_t nadm = 0, write = wend;
+        for (size_t i = wend; i-- > ph;) {
+          if (take[i - ph]) {
+            g_ws.h_admit[nadm++] = pending[i];
+          } else if (--write != i) {
+            pending[write] = pending[i];
+            std::memcpy(&pneed[write * kNeed], &pneed[i * kNeed], sizeof(uint32_t) * kNeed);
+          }
+        }
+        ph = write;
         HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
                               hipMemcpyHostToDevice, g_ws.copy_stream));
-        if (!admitted.empty())
-          HIP_OK(hipMemcpyAsync(g_ws.resume[0], admitted.data(), sizeof(Resume) * admitted.size(),
+        if (nadm)
+          HIP_OK(hipMemcpyAsync(g_ws.resume[0], g_ws.h_admit, sizeof(Resume) * nadm,
                                 hipMemcpyHostToDevice, g_ws.copy_stream));
         HIP_OK(hipStreamSynchronize(g_ws.copy_stream));
         link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
                        .count();
-        pending.swap(rest);
         // next round: continue the edge scan and the task queue, resume the
         // admitted pieces (they may split and defer again)
         unsigned long long nc[5] = {cur8[0], tasks_left ? tasks_done : 0ull,
@@ -2353,13 +2481,19 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
         hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_used,
                            (pg->nslots + 3) / 4, 0u);
         L.rin = (const Resume*)g_ws.resume[0];
-        L.nrin = admitted.size();
+        L.nrin = nadm;
         L.rout = (Resume*)g_ws.resume[1];
+        const auto k0 = std::chrono::steady_clock::now();
         if (edges_left) hipLaunchKernelGGL(k_match_edges<true>, dim3(waves), dim3(kWave), 0, st, L);
         if (L.nrin) hipLaunchKernelGGL(k_resume<true>, dim3(waves), dim3(kWave), 0, st, L);
-        if (L.tasks) hipLaunchKernelGGL(k_match_tasks<true>, dim3(waves), dim3(kWave), 0, st, L);
+        // tasks left over, or written this round by the edge scan or by
+        // resumed depth-0 pieces
+        if (L.tasks && (tasks_left || L.nrin || edges_left))
+          hipLaunchKernelGGL(k_match_tasks<true>, dim3(waves), dim3(kWave), 0, st, L);
         HIP_OK(hipGetLastError());
         HIP_OK(hipStreamSynchronize(st));
+        round_kernel_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - k0).count();
         rounds++;
         if (rounds > 1000000) {
           set_error("out-of-core: no progress");
@@ -2424,6 +2558,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     rq->chunk = (int32_t)chunk;
     rq->link_bytes = link_bytes;
     rq->link_ms = link_ms;
+    rq->round_kernel_ms = round_kernel_ms;
     rq->misses = misses;
     rq->deferred = deferred;
     rq->rounds = rounds;

@@ -129,11 +129,11 @@ int autotune(pm_graph* g, pm_pattern* const* p, int n, const pm_opts* base,
   double cur = first.ms;
   const int chunks[] = {0, 4, 16, 64};
   const int splits[] = {0, 2048, 32768, -1};
-  const int hubs[] = {0, 4, 32};
+  const int hubs[] = {0, 2, 4, 32};
   for (int pass = 0; pass < 2; pass++) {
     for (int knob = 0; knob < 3; knob++) {
       const int* vals = knob == 0 ? chunks : knob == 1 ? splits : hubs;
-      const int nv = knob == 2 ? 3 : 4;
+      const int nv = 4;
       for (int v = 0; v < nv; v++) {
         pm_opts c = o;
         int32_t* field = knob == 0 ? &c.chunk : knob == 1 ? &c.split : &c.hub_factor;
