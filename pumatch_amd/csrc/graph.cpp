@@ -330,7 +330,8 @@ int write_graph_files(const std::string& prefix, const HostGraph& g) {
       set_error("cannot write " + prefix + ".meta.txt");
       return PM_ERR_IO;
     }
-    meta << g.n << " " << g.m << " 4 " << g.max_degree << "\n";
+    // one field per line, as graph_converter/compactify.cpp:45-48 writes it
+    meta << g.n << "\n" << g.m << "\n4\n" << g.max_degree << "\n";
   }
   if (!write_exact(prefix + ".vertex.bin", g.rowptr.data(), sizeof(uint64_t) * (g.n + 1)) ||
       !write_exact(prefix + ".edge.bin", g.colidx.data(), sizeof(uint32_t) * g.m)) {

@@ -133,3 +133,34 @@ def test_cli_usage(root):
     assert os.path.exists(cli)
     r = subprocess.run([cli], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_snap_converter(tmp_path):
+    """bin/convert: ids renumbered by first appearance, both directions,
+    rows sorted and deduplicated, self-loops dropped, labels rand() % 10
+    after srand(0) as the reference's labelize step."""
+    import ctypes
+    import subprocess
+
+    snap = tmp_path / "snap.txt"
+    snap.write_text("# comment\n10 20\n20 30\n\n30 10\n10 10\n20 10\n40 40\n50 20\n")
+    prefix = str(tmp_path / "g")
+    exe = os.path.join(os.path.dirname(pm.CLI_PATH), "convert")
+    r = subprocess.run([exe, str(snap), prefix], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert open(prefix + ".meta.txt").read().split() == ["5", "8", "4", "3"]
+    g = pm.Graph.open(prefix, labels=True, device=False)
+    rp, ci, lb = g.csr()
+    # 10->0, 20->1, 30->2, 40->3, 50->4
+    rows = [sorted(ci[rp[v]:rp[v + 1]].tolist()) for v in range(5)]
+    assert rows == [[1, 2], [0, 2, 4], [0, 1], [], [1]]
+    libc = ctypes.CDLL(None)
+    libc.srand(0)
+    assert lb.tolist() == [libc.rand() % 10 for _ in range(5)]
+    r = subprocess.run([exe, str(snap), prefix, "--keep-self-loops", "--labels", "3", "--seed", "2"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    g = pm.Graph.open(prefix, labels=True, device=False)
+    rp, ci, lb = g.csr()
+    assert sorted(ci[rp[0]:rp[1]].tolist()) == [0, 1, 2] and ci[rp[3]:rp[4]].tolist() == [3]
+    assert lb.max() < 3
