@@ -1174,6 +1174,7 @@ struct Launch {
   uint64_t nrin;
   Resume* rout;  // deferred pieces for the next round
   uint64_t rout_cap;
+  uint64_t soft;  // beyond this many new pieces, searches stop at their first miss
   // learned job order: position s lies in the row of root_order[i] where
   // root_start[i] <= s < root_start[i+1] (null: CSR order)
   const uint32_t* root_order;
@@ -1192,7 +1193,7 @@ __device__ __forceinline__ DeferOut defer_out(const Launch& L) {
   O.buf = L.rout;
   O.count = &L.cursor[4];
   O.cap = L.rout_cap;
-  O.soft = L.rout_cap / 2;
+  O.soft = L.soft;
   return O;
 }
 
@@ -1482,6 +1483,66 @@ __global__ __launch_bounds__(256) void k_copy_packages(const uint32_t* __restric
   }
 }
 
+// Pending pieces stay in HBM between rounds.  Needs of a piece: the
+// packages of its path rows phi[0..depth] and the package that stopped it.
+constexpr int kNeed = kMaxDepth + 2;  // count + packages
+
+__device__ __forceinline__ int piece_needs(const Resume& d, const uint32_t* pkg_of, uint32_t* q) {
+  int c = 0;
+  for (int j = 0; j <= d.depth && j < kMaxDepth; j++) q[c++] = pkg_of[d.phi[j]];
+  q[c++] = d.blk;
+  return c;
+}
+
+// needs of pieces [0, n) for the host's admission (row i: count, packages)
+__global__ void k_piece_needs(const Resume* __restrict__ P, uint64_t n,
+                              const uint32_t* __restrict__ pkg_of, uint32_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t q[kNeed];
+    const int c = piece_needs(P[i], pkg_of, q);
+    out[i * kNeed] = (uint32_t)c;
+    for (int j = 0; j < c; j++) out[i * kNeed + 1 + j] = q[j];
+  }
+}
+
+// Split the pending pieces: those whose needs are all resident go to `run`
+// (resumed this round), the others to `keep`.  Counts in cnt[0] (run) and
+// cnt[1] (keep); one atomic per wave for each.  Pieces keep their relative
+// order within a wave only; the rounds' progress does not depend on it.
+__global__ __launch_bounds__(256) void k_piece_select(const Resume* __restrict__ P, uint64_t n,
+                                                      const uint32_t* __restrict__ pkg_of,
+                                                      const int32_t* __restrict__ pkg_slot,
+                                                      Resume* __restrict__ run,
+                                                      Resume* __restrict__ keep,
+                                                      unsigned long long* cnt) {
+  const uint32_t lane = threadIdx.x % kWave;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x - lane); b < n; b += stride) {
+    const uint64_t i = b + lane;
+    bool live = i < n, here = false;
+    Resume d;
+    if (live) {
+      d = P[i];
+      uint32_t q[kNeed];
+      const int c = piece_needs(d, pkg_of, q);
+      here = true;
+      for (int j = 0; j < c; j++) here = here && pkg_slot[q[j]] >= 0;
+    }
+    const uint64_t mr = __ballot(live && here), mk = __ballot(live && !here);
+    unsigned long long br = 0, bk = 0;
+    if (lane == 0) {
+      if (mr) br = atomicAdd(&cnt[0], (unsigned long long)__popcll(mr));
+      if (mk) bk = atomicAdd(&cnt[1], (unsigned long long)__popcll(mk));
+    }
+    br = __shfl(br, 0);
+    bk = __shfl(bk, 0);
+    if (live && here) run[br + __popcll(mr & lt)] = d;
+    if (live && !here) keep[bk + __popcll(mk & lt)] = d;
+  }
+}
+
 #define HIP_OK(call)                                                \
   do {                                                              \
     hipError_t _e = (call);                                         \
@@ -1510,10 +1571,12 @@ struct Workspace {
   size_t table_bytes = 0;
   void* resume[2] = {nullptr, nullptr};
   size_t resume_bytes[2] = {0, 0};
-  Resume* h_fresh = nullptr;  // pinned staging: pieces device -> host
-  size_t h_fresh_cap = 0;
-  Resume* h_admit = nullptr;  // pinned staging: admitted pieces host -> device
-  size_t h_admit_cap = 0;
+  void* pend[2] = {nullptr, nullptr};  // pending pieces between rounds
+  size_t pend_bytes[2] = {0, 0};
+  void* d_needs = nullptr;             // needs of the admission window
+  size_t d_needs_bytes = 0;
+  uint32_t* h_needs = nullptr;         // pinned copy of the same
+  size_t h_needs_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, eve = nullptr, evc = nullptr;
   int cus = 0;
   int resident = 0;
@@ -1540,13 +1603,26 @@ int ensure_workspace(int device) {
   return PM_OK;
 }
 
-int grow_pinned(Resume** p, size_t* have, size_t need) {
+int grow_pinned_u32(uint32_t** p, size_t* have, size_t need) {
   if (need <= *have) return PM_OK;
-  const size_t n = std::max(need, *have * 2);
   if (*p) HIP_OK(hipHostFree(*p));
   *p = nullptr;
   *have = 0;
-  HIP_OK(hipHostMalloc((void**)p, sizeof(Resume) * n, hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)p, sizeof(uint32_t) * need, hipHostMallocDefault));
+  *have = need;
+  return PM_OK;
+}
+
+// grow a device buffer to `need` bytes, keeping its first `keep` bytes
+int grow_keep(void** p, size_t* have, size_t need, size_t keep, hipStream_t st) {
+  if (need <= *have) return PM_OK;
+  const size_t n = std::max(need, *have * 2);
+  void* q = nullptr;
+  HIP_OK(hipMalloc(&q, n));
+  if (keep) HIP_OK(hipMemcpyAsync(q, *p, keep, hipMemcpyDeviceToDevice, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (*p) HIP_OK(hipFree(*p));
+  *p = q;
   *have = n;
   return PM_OK;
 }
@@ -2205,6 +2281,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   if (paged && opts->package_use) use_acc.assign(pg->layout.count(), 0);
   L.task_cap = g_ws.task_cap;
   L.rout_cap = rcap;
+  L.soft = rcap / 2;
   const int reps = opts->repeats > 0 ? opts->repeats : 1;
   double sum_ms = 0, min_ms = 1e30, phase[3] = {0, 0, 0}, link_ms = 0, round_kernel_ms = 0;
   uint64_t link_bytes = 0, misses = 0, deferred = 0;
@@ -2306,15 +2383,19 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       // so the rounds terminate; nothing is dropped, so counts are exact.
       const PackageLayout& PL = pg->layout;
       const uint64_t span = (uint64_t)npat * nchunk;  // phase-1 units
-      const size_t admit_cap = 1u << 18;  // pieces resumed per round
-      // pending pieces, oldest first, live in pending[ph..]; a round looks
-      // at no more than 2 * admit_cap of them
-      // needs of a piece: the packages of its path rows and the package that
-      // stopped it, kNeed - 1 entries at most, computed once on arrival
-      constexpr int kNeed = kMaxDepth + 2;
-      std::vector<Resume> pending;
-      std::vector<uint32_t> pneed;  // kNeed per pending piece: count, packages
-      size_t ph = 0;
+      // Pending pieces stay in HBM (pend[pc][0..npend)); each round the
+      // host reads the needs of the first kWindow of them, admits those
+      // oldest first while the union of their needs fits the cache, loads
+      // that set, and the device moves every pending piece whose needs are
+      // resident to the resume buffer.
+      constexpr uint64_t kWindow = 1u << 16;
+      constexpr uint64_t kPendMax = 1ull << 26;  // pieces held between rounds
+      uint64_t npend = 0;
+      int pc = 0;
+      rc = grow_pinned_u32(&g_ws.h_needs, &g_ws.h_needs_cap, kWindow * kNeed);
+      if (rc) return rc;
+      rc = grow(&g_ws.d_needs, &g_ws.d_needs_bytes, sizeof(uint32_t) * kWindow * kNeed);
+      if (rc) return rc;
       for (;;) {
         unsigned long long cur8[8], mc = 0;
         HIP_OK(hipMemcpy(cur8, g_ws.cursor, sizeof(cur8), hipMemcpyDeviceToHost));
@@ -2331,21 +2412,22 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
         misses += mc;
         deferred += nd;
         if (nd) {
-          rc = grow_pinned(&g_ws.h_fresh, &g_ws.h_fresh_cap, nd);
-          if (rc) return rc;
-          HIP_OK(hipMemcpy(g_ws.h_fresh, g_ws.resume[1], sizeof(Resume) * nd,
-                           hipMemcpyDeviceToHost));
-          pending.insert(pending.end(), g_ws.h_fresh, g_ws.h_fresh + nd);
-          const size_t base = pneed.size();
-          pneed.resize(base + nd * kNeed);
-          for (uint64_t i = 0; i < nd; i++) {
-            const Resume& d = g_ws.h_fresh[i];
-            uint32_t* q = &pneed[base + i * kNeed];
-            int c = 0;
-            for (int j = 0; j <= d.depth && j < kMaxDepth; j++) q[1 + c++] = PL.pkg_of[d.phi[j]];
-            q[1 + c++] = d.blk;
-            q[0] = (uint32_t)c;
+          if (npend + nd > kPendMax) {
+            set_error("out-of-core: too many deferred pieces pending");
+            return PM_ERR_OVERFLOW;
           }
+          // pending buffers hold npend + nd; the run buffer may take all
+          for (int i = 0; i < 2; i++) {
+            rc = grow_keep(&g_ws.pend[i], &g_ws.pend_bytes[i], sizeof(Resume) * (npend + nd),
+                           i == pc ? sizeof(Resume) * npend : 0, st);
+            if (rc) return rc;
+          }
+          rc = grow_keep(&g_ws.resume[0], &g_ws.resume_bytes[0], sizeof(Resume) * (npend + nd), 0,
+                         st);
+          if (rc) return rc;
+          HIP_OK(hipMemcpyAsync((Resume*)g_ws.pend[pc] + npend, g_ws.resume[1],
+                                sizeof(Resume) * nd, hipMemcpyDeviceToDevice, st));
+          npend += nd;
         }
         const bool edges_left = cur8[0] < span;
         const uint64_t tasks_done = std::min(cur8[1], cur8[2]);
@@ -2363,23 +2445,25 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
             }
           }
         }
-        if (ph == pending.size() && !edges_left && !tasks_left) break;
+        if (npend == 0 && !edges_left && !tasks_left) break;
         pg->policy.touched(used.data(), (uint32_t)pg->nslots);
-        if (ph > 0 && ph * 2 >= pending.size()) {
-          pending.erase(pending.begin(), pending.begin() + (std::ptrdiff_t)ph);
-          pneed.erase(pneed.begin(), pneed.begin() + (std::ptrdiff_t)(ph * kNeed));
-          ph = 0;
+        // admission: the oldest pieces of the window while their needs fit
+        const uint64_t nw = std::min(npend, kWindow);
+        if (nw) {
+          hipLaunchKernelGGL(k_piece_needs, dim3((unsigned)std::min<uint64_t>((nw + 255) / 256, 1024)),
+                             dim3(256), 0, st, (const Resume*)g_ws.pend[pc], nw, pg->d_pkg_of,
+                             (uint32_t*)g_ws.d_needs);
+          HIP_OK(hipMemcpyAsync(g_ws.h_needs, g_ws.d_needs, sizeof(uint32_t) * nw * kNeed,
+                                hipMemcpyDeviceToHost, st));
+          HIP_OK(hipStreamSynchronize(st));
         }
-        const size_t wend = std::min(pending.size(), ph + admit_cap);
-        // admission: oldest pieces first while their needs fit the cache
         std::vector<uint8_t> in_set(np, 0);
         std::vector<uint32_t> want;
         uint64_t set_slots = 0;
         size_t nadmit = 0;
-        for (size_t i = ph; i < wend; i++) {
-          if (nadmit >= admit_cap) break;
-          const uint32_t* need = &pneed[i * kNeed + 1];
-          const int nneed = (int)pneed[i * kNeed];
+        for (uint64_t i = 0; i < nw; i++) {
+          const uint32_t* need = &g_ws.h_needs[i * kNeed + 1];
+          const int nneed = (int)g_ws.h_needs[i * kNeed];
           uint64_t extra = 0;
           for (int q = 0; q < nneed; q++) {
             bool dup = in_set[need[q]] != 0;
@@ -2395,7 +2479,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
           set_slots += extra;
           nadmit++;
         }
-        if (nadmit == 0 && ph < pending.size() && !edges_left && !tasks_left) {
+        if (nadmit == 0 && npend && !edges_left && !tasks_left) {
           set_error("out-of-core: a deferred search needs more packages than the cache holds");
           return PM_ERR_OVERFLOW;
         }
@@ -2435,43 +2519,29 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
             link_bytes += b;
           }
         }
-        // resume every piece of the window whose needs are now resident (the
-        // admitted ones and any other that finds its packages in the cache);
-        // the others stay pending in their order
-        std::vector<uint8_t> take(wend - ph, 0);
-        size_t ntake = 0;
-        for (size_t i = ph; i < wend && ntake < admit_cap; i++) {
-          const uint32_t* need = &pneed[i * kNeed + 1];
-          const int nneed = (int)pneed[i * kNeed];
-          bool here = true;
-          for (int q = 0; q < nneed && here; q++) here = pg->pkg_slot[need[q]] >= 0;
-          if (here) {
-            take[i - ph] = 1;
-            ntake++;
-          }
-        }
-        rc = grow_pinned(&g_ws.h_admit, &g_ws.h_admit_cap, std::max<size_t>(ntake, 1));
-        if (rc) return rc;
-        size_t nadm = 0, write = wend;
-        for (size_t i = wend; i-- > ph;) {
-          if (take[i - ph]) {
-            g_ws.h_admit[nadm++] = pending[i];
-          } else if (--write != i) {
-            pending[write] = pending[i];
-            std::memcpy(&pneed[write * kNeed], &pneed[i * kNeed], sizeof(uint32_t) * kNeed);
-          }
-        }
-        ph = write;
         HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
                               hipMemcpyHostToDevice, g_ws.copy_stream));
-        if (nadm)
-          HIP_OK(hipMemcpyAsync(g_ws.resume[0], g_ws.h_admit, sizeof(Resume) * nadm,
-                                hipMemcpyHostToDevice, g_ws.copy_stream));
         HIP_OK(hipStreamSynchronize(g_ws.copy_stream));
         link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
                        .count();
+        // every pending piece whose needs are now resident runs this round
+        uint64_t nrun = 0;
+        if (npend) {
+          hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor + 6, 2, 0ull);
+          hipLaunchKernelGGL(k_piece_select,
+                             dim3((unsigned)std::min<uint64_t>((npend + 255) / 256, 16384)),
+                             dim3(256), 0, st, (const Resume*)g_ws.pend[pc], npend, pg->d_pkg_of,
+                             pg->d_pkg_slot, (Resume*)g_ws.resume[0], (Resume*)g_ws.pend[1 - pc],
+                             g_ws.cursor + 6);
+          unsigned long long sel[2];
+          HIP_OK(hipMemcpyAsync(sel, g_ws.cursor + 6, sizeof(sel), hipMemcpyDeviceToHost, st));
+          HIP_OK(hipStreamSynchronize(st));
+          nrun = sel[0];
+          npend = sel[1];
+          pc = 1 - pc;
+        }
         // next round: continue the edge scan and the task queue, resume the
-        // admitted pieces (they may split and defer again)
+        // selected pieces (they may split and defer again)
         unsigned long long nc[5] = {cur8[0], tasks_left ? tasks_done : 0ull,
                                     tasks_left ? cur8[2] : 0ull, 0ull, 0ull};
         HIP_OK(hipMemcpyAsync(g_ws.cursor, nc, sizeof(nc), hipMemcpyHostToDevice, st));
@@ -2481,7 +2551,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
         hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_used,
                            (pg->nslots + 3) / 4, 0u);
         L.rin = (const Resume*)g_ws.resume[0];
-        L.nrin = nadm;
+        L.nrin = nrun;
         L.rout = (Resume*)g_ws.resume[1];
         const auto k0 = std::chrono::steady_clock::now();
         if (edges_left) hipLaunchKernelGGL(k_match_edges<true>, dim3(waves), dim3(kWave), 0, st, L);
