@@ -221,6 +221,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and learned objects")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched object")
     ap.add_argument("--no-tune", action="store_true", help="time default launch options")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
@@ -300,9 +301,10 @@ def main():
         cpu = cpu_baseline(g, text, first.per_edge, count, args.cpu_seconds, threads)
 
     extras = {}
-    if not args.no_extras:
+    if not args.no_extras and not args.no_batched:
         print("[bench] batched: 64 patterns, plain sequential vs tuned batch", flush=True)
         extras["batched"] = batched_section(pm)
+    if not args.no_extras:
         print("[bench] learned paging: train on seeds 7-9, evaluate on seed 10", flush=True)
         extras["learned"] = learned_section(pm)
 
