@@ -24,10 +24,11 @@ Two more objects compare plain and learned+batched schedules:
     0.3-0.8, seed 6) on RMAT scale 14 with 8 labels (seed 5): plain =
     one pm_match per pattern in file order with default options; batched =
     one pm_match_batch launch with autotuned orders and granularities.
-  "learned": out-of-core runs on a Chung-Lu graph (50k vertices, exponent
-    2.2, max degree 500, mean 20, ids permuted) with 40% of its rows in
-    the HBM cache: plain LRU vs the MLP trained on traces of seeds 7-9 and
-    evaluated on seed 10 (learned prefetch + eviction).
+  "learned": the MLP trained on traces of seeds 7-9 and evaluated on seed
+    10: learned job order on permuted RMAT scale 15 (resident), and learned
+    prefetch + eviction, alone and in one batched launch, out of core on
+    permuted Chung-Lu graphs (50k vertices) with 40% of the rows in HBM,
+    each against the plain schedule.
 """
 import argparse
 import json
@@ -132,65 +133,119 @@ def batched_section(pm):
 
 
 def learned_section(pm):
+    """Train the MLP on traces of plain runs over graphs of seeds 7-9 and
+    evaluate on seed 10: (1) learned job order on permuted RMAT scale 15,
+    whole chip with default granularities and with 1024 waves and no
+    splitting; (2) learned package ranking (prefetch + eviction) out of core
+    on permuted Chung-Lu graphs against LRU; (3) one batched out-of-core
+    launch of the same patterns with the learned policy against the plain
+    runs one by one."""
     import numpy as np
 
     from pumatch_amd.workloads import permuted
 
+    def text(k, edges, labels=None):
+        return pattern_text(k, edges, labels)
+
+    tri = pm.Pattern.parse(text(3, [(0, 1), (1, 2), (0, 2)]), False)
+    k4 = pm.Pattern.parse(text(4, [(a, b) for a in range(4) for b in range(a)]), False)
+    c1 = pm.Pattern.parse(text(7, C1_EDGES), True)
+    out = {}
+
+    # (1) learned job order
+    def rmat(seed):
+        g0 = pm.Graph.rmat(15, 16, seed=seed, device=False)
+        g = permuted(g0, seed + 100)
+        g.set_uniform_labels(4, seed)
+        return g
+
+    pats = [("k4", k4), ("triangle", tri), ("c1", c1)]
+    Xs, Ys, Ws = [], [], []
+    for seed in (7, 8, 9):
+        g = rmat(seed)
+        rw = np.zeros(g.n)
+        for _, p in pats:
+            rw += pm.match(g, p, traces=True).root_work
+        Xs.append(pm.features(g))
+        Ys.append(np.stack([np.log2(1 + rw), np.zeros(g.n)], 1))
+        Ws.append(np.stack([np.ones(g.n), np.zeros(g.n)], 1))
+    m = pm.Model(1)
+    t0 = time.perf_counter()
+    loss = m.train(np.concatenate(Xs), np.concatenate(Ys).astype(np.float32),
+                   np.concatenate(Ws).astype(np.float32), epochs=30, lr=3e-3)
+    train_s = time.perf_counter() - t0
+    g = rmat(10)
+    order = {"workload": ("rmat15-perm: RMAT scale 15, edge factor 16, ids permuted, 4 labels; "
+                          "unlabeled K4 and triangle, labeled K4 + path"),
+             "train": f"root-work traces of plain runs on seeds 7-9, {train_s:.2f} s on the GPU, "
+                      f"final loss {loss:.3f}", "eval_seed": 10}
+    for tag, kw in (("default", {}), ("1024_waves_no_split", dict(waves=1024, split=-1))):
+        tp = tl = 0.0
+        per = {}
+        for name, p in pats:
+            a = pm.match(g, p, repeats=5, **kw)
+            b = pm.match(g, p, repeats=5, model=m, learned=pm.PM_LEARN_ORDER, **kw)
+            assert a.count == b.count
+            per[name] = [round(a.min_ms, 3), round(b.min_ms, 3)]
+            tp += a.min_ms
+            tl += b.min_ms
+        order[tag] = {"plain_ms": round(tp, 3), "learned_ms": round(tl, 3),
+                      "speedup": round(tp / tl, 3), "per_pattern_plain_learned_ms": per}
+    out["order"] = order
+
+    # (2) learned package ranking out of core, (3) batched
     N, PKG = 50000, 16384
 
-    def graph(seed):
+    def chunglu(seed):
         g0 = pm.Graph.chung_lu(N, 2.2, N // 100, 20, seed=seed, device=False)
         g = permuted(g0, seed + 100)
         g.set_uniform_labels(8, seed)
         return g
 
-    def text(k, edges):
-        return pattern_text(k, edges)
-
-    pats = [("triangle", pm.Pattern.parse(text(3, [(0, 1), (1, 2), (0, 2)]), False)),
-            ("k4", pm.Pattern.parse(text(4, [(a, b) for a in range(4) for b in range(a)]), False))]
+    ppats = [("triangle", tri), ("k4", k4)]
     Xs, Ys = [], []
     for seed in (7, 8, 9):
-        g = graph(seed)
+        g = chunglu(seed)
         cache = int(g.m * 4 * 0.4) // PKG * PKG
         rw, pu = np.zeros(g.n), np.zeros(g.n)
-        for _, p in pats:
+        for _, p in ppats:
             r = pm.match(g, p, traces=True, cache_bytes=cache, package_bytes=PKG)
             rw += r.root_work
             pu += r.package_use
         Xs.append(pm.features(g))
         Ys.append(np.stack([np.log2(1 + rw), np.log2(1 + pu)], 1))
-    X = np.concatenate(Xs)
-    Y = np.concatenate(Ys).astype(np.float32)
-    m = pm.Model(1)
-    t0 = time.perf_counter()
-    loss = m.train(X, Y, epochs=30, lr=3e-3)
-    train_s = time.perf_counter() - t0
-    g = graph(10)
+    mp = pm.Model(1)
+    mp.train(np.concatenate(Xs), np.concatenate(Ys).astype(np.float32), epochs=30, lr=3e-3)
+    g = chunglu(10)
     cache = int(g.m * 4 * 0.4) // PKG * PKG
-    out = {"workload": (f"chunglu50k: Chung-Lu n={N}, exponent 2.2, max degree {N // 100}, mean 20, "
-                        f"ids permuted; {cache} B HBM cache of {PKG} B packages (40% of rows); "
-                        "unlabeled triangle and K4"),
-           "train": f"traces of plain LRU runs on seeds 7-9 ({X.shape[0]} roots), "
-                    f"{train_s:.2f} s on the GPU, final loss {loss:.3f}",
-           "eval_seed": 10, "patterns": {}}
+    paging = {"workload": (f"chunglu50k-perm: Chung-Lu n={N}, exponent 2.2, max degree {N // 100}, "
+                           f"mean 20, ids permuted; {cache} B HBM cache of {PKG} B packages "
+                           "(40% of rows); unlabeled triangle and K4"), "eval_seed": 10,
+              "patterns": {}}
     tp = tl = 0.0
-    for name, p in pats:
+    for name, p in ppats:
         a = pm.match(g, p, cache_bytes=cache, package_bytes=PKG)
         b = pm.match(g, p, cache_bytes=cache, package_bytes=PKG, policy="learned", prefetch=True,
-                     model=m)
+                     model=mp)
         assert a.count == b.count
-        out["patterns"][name] = {"plain_ms": round(a.kernel_ms, 2), "learned_ms": round(b.kernel_ms, 2),
-                                 "plain_rounds": a.rounds, "learned_rounds": b.rounds,
-                                 "plain_link_MB": round(a.link_bytes / 1e6, 2),
-                                 "learned_link_MB": round(b.link_bytes / 1e6, 2),
-                                 "plain_link_GBps": round(a.link_bytes / max(a.link_ms, 1e-9) / 1e6, 2),
-                                 "count": a.count}
+        paging["patterns"][name] = {
+            "plain_ms": round(a.kernel_ms, 2), "learned_ms": round(b.kernel_ms, 2),
+            "plain_rounds": a.rounds, "learned_rounds": b.rounds,
+            "plain_link_MB": round(a.link_bytes / 1e6, 2), "learned_link_MB": round(b.link_bytes / 1e6, 2),
+            "link_GBps_during_copies": round(a.link_bytes / max(a.link_ms, 1e-9) / 1e6, 2),
+            "count": a.count}
         tp += a.kernel_ms
         tl += b.kernel_ms
-    out["plain_ms"] = round(tp, 2)
-    out["learned_ms"] = round(tl, 2)
-    out["speedup"] = round(tp / tl, 3)
+    paging["plain_ms"] = round(tp, 2)
+    paging["learned_ms"] = round(tl, 2)
+    paging["speedup"] = round(tp / tl, 3)
+    ps = [p for _, p in ppats]
+    rb = pm.match_batch(g, ps, cache_bytes=cache, package_bytes=PKG, policy="learned", prefetch=True,
+                        model=mp, learned=pm.PM_LEARN_ORDER | pm.PM_LEARN_PAGING)
+    assert [r.count for r in rb] == [paging["patterns"][n]["count"] for n, _ in ppats]
+    paging["learned_batched_ms"] = round(rb[0].kernel_ms, 2)
+    paging["learned_batched_speedup_vs_plain_sequential"] = round(tp / rb[0].kernel_ms, 3)
+    out["paging"] = paging
     return out
 
 
