@@ -281,8 +281,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: the matcher shards by job ranges across GPUs "
-              "(see DESIGN.md), but only --gpus 1 is run here", file=sys.stderr)
+        print("bench.py measures one MI355X (--gpus 1): jobs are independent CSR-edge ranges, so "
+              "the matcher would shard by edge ranges with one u64 sum per pattern at the end, "
+              "but several GPUs are not run here", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()

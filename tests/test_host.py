@@ -164,3 +164,45 @@ def test_snap_converter(tmp_path):
     rp, ci, lb = g.csr()
     assert sorted(ci[rp[0]:rp[1]].tolist()) == [0, 1, 2] and ci[rp[3]:rp[4]].tolist() == [3]
     assert lb.max() < 3
+
+
+def test_model_parameters_round_trip_and_init():
+    """pm_model_create / get / set on the host: He-uniform weights within
+    their bounds, zero biases, exact round trip, layout size."""
+    from oracle import mlp
+
+    m = pm.Model(5)
+    w = m.params
+    assert w.size == pm.N_PARAMS == mlp.NPARAMS
+    W1 = w[mlp.OFF["W1"][0]:mlp.OFF["W1"][0] + mlp.OFF["W1"][1]]
+    W2 = w[mlp.OFF["W2"][0]:mlp.OFF["W2"][0] + mlp.OFF["W2"][1]]
+    assert np.abs(W1).max() <= np.sqrt(6 / 16) and np.abs(W2).max() <= np.sqrt(6 / 64)
+    assert np.abs(W1).std() > 0.1
+    for name in ("b1", "b2", "b3"):
+        o, n = mlp.OFF[name]
+        assert not np.any(w[o:o + n])
+    assert np.array_equal(pm.Model(5).params, w)  # seeded
+    assert not np.array_equal(pm.Model(6).params, w)
+    v = np.random.default_rng(0).normal(size=pm.N_PARAMS).astype(np.float32)
+    m.params = v
+    assert np.array_equal(m.params, v)
+
+
+def test_batch_and_tune_argument_errors():
+    """Argument checks of the batched, tuning and model entry points run
+    before any device work."""
+    L = pm.lib()
+    g = pm.Graph.rmat(6, 4, seed=1, device=False)
+    p = pm.Pattern.parse(g_text(3, edges_to_adj(3, [(0, 1), (1, 2)])), False)
+    rs = (pm.Result * 2)()
+    assert L.pm_match_batch(None, None, 1, None, rs) == pm.PM_ERR_ARG
+    assert L.pm_match_batch(g._h, (pm.C.c_void_p * 1)(p._h.value), -1, None, rs) == pm.PM_ERR_ARG
+    assert L.pm_match_batch(g._h, (pm.C.c_void_p * 2)(p._h.value, None), 2, None, rs) == pm.PM_ERR_ARG
+    assert L.pm_match_batch(g._h, (pm.C.c_void_p * 1)(p._h.value), 0, None, rs) == pm.PM_OK
+    best, rep = pm.Opts(), pm.TuneReport()
+    hs = (pm.C.c_void_p * 1)(p._h.value)
+    assert L.pm_autotune(g._h, hs, 0, None, None, pm.C.byref(best), pm.C.byref(rep)) == pm.PM_ERR_ARG
+    assert L.pm_autotune(g._h, hs, 1, None, None, None, pm.C.byref(rep)) == pm.PM_ERR_ARG
+    assert L.pm_model_get(None, None) == pm.PM_ERR_ARG
+    assert L.pm_model_predict(None, None, 0, None) == pm.PM_ERR_ARG
+    assert L.pm_model_train(None, None, None, None, 1, None, None) == pm.PM_ERR_ARG
