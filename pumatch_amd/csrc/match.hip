@@ -79,6 +79,10 @@ constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
 #ifndef PM_RESIDENT_WAVES
 #define PM_RESIDENT_WAVES 5
 #endif
+// sorted-set intersection: two 64-element batches per step (1) or one (0)
+#ifndef PM_ISECT_DUAL
+#define PM_ISECT_DUAL 0
+#endif
 constexpr int kWalk = PM_FUSED_WALK;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kMaxDepth = 6;
@@ -308,7 +312,12 @@ __device__ __forceinline__ uint32_t hslot(uint32_t y) {
 // WRITE: store the hits in order at out.  LABEL: keep y only when bit lab[y]
 // of labmask is set.  WEIGHT: also sum wt[y] (1 when wt is null) over the
 // hits into *wsum.
-template <bool WRITE, bool LABEL, bool WEIGHT>
+// resident kernels interleave two batches; the out-of-core ones, which run
+// closer to their register limit, do not
+template <bool PG>
+constexpr bool kDual = !PG && PM_ISECT_DUAL;
+
+template <bool WRITE, bool LABEL, bool WEIGHT, bool DUAL = false>
 __device__ uint32_t wave_isect(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
                                uint32_t labmask, const uint8_t* __restrict__ lab,
                                const unsigned long long* __restrict__ wt, uint32_t* out,
@@ -325,6 +334,58 @@ __device__ uint32_t wave_isect(const uint32_t* a, uint32_t na, const uint32_t* b
   uint32_t cnt = 0, lo = 0;
   unsigned long long ws = 0;
   const uint64_t lt = (1ull << lane) - 1ull;
+  if (DUAL) {
+  // two batches of 64 per step: their binary searches (both from the
+  // window start lo) are interleaved so their loads are in flight together
+  for (uint32_t base = 0; base < na && lo < nb; base += 2 * kWave) {
+    const uint32_t i0 = base + lane, i1 = base + kWave + lane;
+    uint32_t x0 = 0, x1 = 0, p0 = lo, p1 = lo, n0 = 0, n1 = 0;
+    if (i0 < na) {
+      x0 = a[i0];
+      n0 = nb - lo;
+    }
+    if (i1 < na) {
+      x1 = a[i1];
+      n1 = nb - lo;
+    }
+    while (n0 | n1) {
+      const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
+      const uint32_t v0 = n0 ? b[p0 + h0] : 0u, v1 = n1 ? b[p1 + h1] : 0u;
+      if (n0) {
+        if (v0 < x0) {
+          p0 += h0 + 1;
+          n0 -= h0 + 1;
+        } else {
+          n0 = h0;
+        }
+      }
+      if (n1) {
+        if (v1 < x1) {
+          p1 += h1 + 1;
+          n1 -= h1 + 1;
+        } else {
+          n1 = h1;
+        }
+      }
+    }
+    bool hit0 = i0 < na && p0 < nb && b[p0] == x0;
+    bool hit1 = i1 < na && p1 < nb && b[p1] == x1;
+    if (LABEL && hit0) hit0 = (labmask >> lab[x0]) & 1u;
+    if (LABEL && hit1) hit1 = (labmask >> lab[x1]) & 1u;
+    if (WEIGHT && hit0) ws += wt ? wt[x0] : 1ull;
+    if (WEIGHT && hit1) ws += wt ? wt[x1] : 1ull;
+    const uint64_t m0 = __ballot(hit0), m1 = __ballot(hit1);
+    if (WRITE && hit0) out[cnt + __popcll(m0 & lt)] = x0;
+    cnt += (uint32_t)__popcll(m0);
+    if (WRITE && hit1) out[cnt + __popcll(m1 & lt)] = x1;
+    cnt += (uint32_t)__popcll(m1);
+    // the last valid element of this step bounds the next window
+    const uint32_t left = na - base;
+    const uint32_t last = (left > kWave ? min(left - kWave, (uint32_t)kWave) : left) - 1u;
+    const uint32_t pl = left > kWave ? p1 : p0;
+    lo = uni((uint32_t)__shfl((int)(i0 < na || i1 < na ? pl : nb), (int)last));
+  }
+  } else {
   for (uint32_t base = 0; base < na && lo < nb; base += kWave) {
     const uint32_t i = base + lane;
     bool hit = false;
@@ -341,6 +402,7 @@ __device__ uint32_t wave_isect(const uint32_t* a, uint32_t na, const uint32_t* b
     cnt += (uint32_t)__popcll(m);
     const uint32_t last = min(na - base, (uint32_t)kWave) - 1u;
     lo = uni((uint32_t)__shfl((int)pos, (int)last));
+  }
   }
   if (WRITE) ct.alg += 4ull * cnt;
   if (WEIGHT) *wsum = wave_sum(ws);
@@ -424,11 +486,11 @@ __device__ bool run_ops(const KGraph& g, const KPlan& P, WaveState& ws, int d, u
       uint32_t* out = arena + (size_t)P.region[op.slot] * cap;
       uint32_t cnt;
       if (P.labeled && !(P.segs && op.seg >= 0))
-        cnt = wave_isect<true, true, false>(a, na, nv, nb, op.labmask, g.lab, nullptr, out, lane,
-                                            ct, nullptr);
+        cnt = wave_isect<true, true, false, kDual<PG>>(a, na, nv, nb, op.labmask, g.lab,
+                                                       nullptr, out, lane, ct, nullptr);
       else
-        cnt = wave_isect<true, false, false>(a, na, nv, nb, 0, g.lab, nullptr, out, lane, ct,
-                                             nullptr);
+        cnt = wave_isect<true, false, false, kDual<PG>>(a, na, nv, nb, 0, g.lab, nullptr, out,
+                                                        lane, ct, nullptr);
       ws.sptr[op.slot] = out;
       ws.slen[op.slot] = uni(cnt);
       ct.nops++;
@@ -544,14 +606,15 @@ __device__ unsigned long long last_count(const KGraph& g, const KPlan& P, WaveSt
     unsigned long long s = 0;
     uint32_t c;
     if (segs)
-      c = wave_isect<false, false, true>(p, np, a, na, 0, g.lab, w2, nullptr, lane, ct, &s);
+      c = wave_isect<false, false, true, kDual<PG>>(p, np, a, na, 0, g.lab, w2, nullptr, lane,
+                                                    ct, &s);
     else
-      c = wave_isect<false, true, true>(p, np, a, na, 1u << P.lab[u], g.lab, w2, nullptr, lane,
-                                        ct, &s);
+      c = wave_isect<false, true, true, kDual<PG>>(p, np, a, na, 1u << P.lab[u], g.lab, w2,
+                                                   nullptr, lane, ct, &s);
     return w2 ? s : (unsigned long long)c;
   }
-  uint32_t cnt = uni(wave_isect<false, false, false>(p, np, a, na, 0, g.lab, nullptr, nullptr,
-                                                     lane, ct, nullptr));
+  uint32_t cnt = uni(wave_isect<false, false, false, kDual<PG>>(p, np, a, na, 0, g.lab, nullptr,
+                                                                nullptr, lane, ct, nullptr));
   const uint32_t xm = P.X[u];
   bool in = false;
   if (lane < (uint32_t)u && ((xm >> lane) & 1u)) {
@@ -892,17 +955,19 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
       unsigned long long s = 0, s1 = 0;
       uint32_t c;
       if (lab && !segs)
-        c = wave_isect<false, true, true>(Pp, nPo, nxo, nxn, 1u << P.lab[u2], g.lab, w2, nullptr,
-                                          lane, ct, &s);
+        c = wave_isect<false, true, true, kDual<PG>>(Pp, nPo, nxo, nxn, 1u << P.lab[u2], g.lab,
+                                                     w2, nullptr, lane, ct, &s);
       else if (lab)
-        c = wave_isect<false, false, true>(Pp, nPo, nxo, nxn, 0, g.lab, w2, nullptr, lane, ct, &s);
+        c = wave_isect<false, false, true, kDual<PG>>(Pp, nPo, nxo, nxn, 0, g.lab, w2, nullptr,
+                                                      lane, ct, &s);
       else
-        c = wave_isect<false, false, false>(Pp, nPo, nxo, nxn, 0, g.lab, nullptr, nullptr, lane,
-                                            ct, nullptr);
+        c = wave_isect<false, false, false, kDual<PG>>(Pp, nPo, nxo, nxn, 0, g.lab, nullptr,
+                                                       nullptr, lane, ct, nullptr);
       if (pair) {
         s1 = c;
-        if (w1) wave_isect<false, false, true>(Pp, nPo, nxo, nxn, 0, g.lab, w1, nullptr, lane, ct,
-                                               &s1);
+        if (w1)
+          wave_isect<false, false, true, kDual<PG>>(Pp, nPo, nxo, nxn, 0, g.lab, w1, nullptr,
+                                                    lane, ct, &s1);
       }
       if (lane == 0) acc += wo * (lab && w2 ? s : (unsigned long long)c) + (pair ? wbo * s1 : 0ull);
     }
@@ -1429,6 +1494,19 @@ __global__ __launch_bounds__(64) void k_fold(FoldArgs A) {
   // colidx 4 B + label 1 B per edge, 8 B per child value gathered
   hits = wave_sum(hits);
   if (lane == 0) atomicAdd(A.bytes, 5ull * edges + 8ull * A.nchild * hits);
+}
+
+// A leaf of a pendant tree over the label-grouped rows: F(v) is the size of
+// the label-`want` part of N(v), read from the row's label offsets.
+__global__ void k_fold_leaf(FoldArgs A) {
+  const KGraph& g = A.g;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < g.n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t* o = g.loff + v * g.nl + A.want;
+    A.out[v] = o[1] - o[0];
+  }
+  // two label offsets per vertex
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.bytes, 8ull * g.n);
 }
 
 // out[v] = prod_c child_c[v]
@@ -2353,6 +2431,10 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       if (!h.n) continue;
       if (j.product) {
         hipLaunchKernelGGL(k_product, dim3(1024), dim3(256), 0, st, h.n, A);
+      } else if (A.nchild == 0 && !paged && kg.nl > 0 && A.want + 1 < kg.nl) {
+        // a leaf with label-grouped rows: the label part's size, no row reads
+        hipLaunchKernelGGL(k_fold_leaf, dim3((unsigned)std::min<uint64_t>((h.n + 255) / 256, 4096)),
+                           dim3(256), 0, st, A);
       } else {
         hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, st, A.out, h.n, 0ull);
         if (h.m) hipLaunchKernelGGL(k_fold, dim3(fold_blocks), dim3(kWave), 0, st, A);
