@@ -2105,6 +2105,217 @@ int learned_order(pm_graph* g, const pm_model* m, const std::vector<float>& pred
   return PM_OK;
 }
 
+// Out-of-core rounds after the first launch.  Deferred searches wait in
+// HBM; each round admits them oldest first while the union of what they
+// need -- the packages of their path rows and of the package that stopped
+// them -- fits the cache, loads exactly that set (pinned against eviction)
+// and resumes every piece whose needs are resident.  Every admitted search
+// then passes its stopping point, so the rounds terminate; nothing is
+// dropped, so counts are exact.
+struct RoundStats {
+  uint64_t misses = 0, deferred = 0, link_bytes = 0;
+  double link_ms = 0, kernel_ms = 0;
+  int rounds = 1;
+};
+
+int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t span, uint64_t rcap,
+               const pm_opts* opts, const std::vector<uint32_t>& by_score,
+               std::vector<uint64_t>& use_acc, RoundStats& rs) {
+  const uint32_t np = pg->layout.count();
+  int rc = PM_OK;
+  const PackageLayout& PL = pg->layout;
+  // Pending pieces stay in HBM (pend[pc][0..npend)); each round the
+  // host reads the needs of the first kWindow of them, admits those
+  // oldest first while the union of their needs fits the cache, loads
+  // that set, and the device moves every pending piece whose needs are
+  // resident to the resume buffer.
+  constexpr uint64_t kWindow = 1u << 16;
+  constexpr uint64_t kPendMax = 1ull << 26;  // pieces held between rounds
+  uint64_t npend = 0;
+  int pc = 0;
+  rc = grow_pinned_u32(&g_ws.h_needs, &g_ws.h_needs_cap, kWindow * kNeed);
+  if (rc) return rc;
+  rc = grow(&g_ws.d_needs, &g_ws.d_needs_bytes, sizeof(uint32_t) * kWindow * kNeed);
+  if (rc) return rc;
+  for (;;) {
+    unsigned long long cur8[8], mc = 0;
+    HIP_OK(hipMemcpy(cur8, g_ws.cursor, sizeof(cur8), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(&mc, pg->d_miss_count, sizeof(mc), hipMemcpyDeviceToHost));
+    const uint64_t nd = cur8[4];
+    if (nd > rcap) {
+      set_error("out-of-core: more deferred pieces in one round than the buffer holds");
+      return PM_ERR_OVERFLOW;
+    }
+    if (cur8[2] > g_ws.task_cap) {
+      set_error("out-of-core: range task buffer overflow");
+      return PM_ERR_OVERFLOW;
+    }
+    rs.misses += mc;
+    rs.deferred += nd;
+    if (nd) {
+      if (npend + nd > kPendMax) {
+        set_error("out-of-core: too many deferred pieces pending");
+        return PM_ERR_OVERFLOW;
+      }
+      // pending buffers hold npend + nd; the run buffer may take all
+      for (int i = 0; i < 2; i++) {
+        rc = grow_keep(&g_ws.pend[i], &g_ws.pend_bytes[i], sizeof(Resume) * (npend + nd),
+                       i == pc ? sizeof(Resume) * npend : 0, st);
+        if (rc) return rc;
+      }
+      rc = grow_keep(&g_ws.resume[0], &g_ws.resume_bytes[0], sizeof(Resume) * (npend + nd), 0,
+                     st);
+      if (rc) return rc;
+      HIP_OK(hipMemcpyAsync((Resume*)g_ws.pend[pc] + npend, g_ws.resume[1],
+                            sizeof(Resume) * nd, hipMemcpyDeviceToDevice, st));
+      npend += nd;
+    }
+    const bool edges_left = cur8[0] < span;
+    const uint64_t tasks_done = std::min(cur8[1], cur8[2]);
+    const bool tasks_left = L.tasks && tasks_done < cur8[2];
+    std::vector<uint8_t> used(pg->nslots);
+    HIP_OK(hipMemcpy(used.data(), pg->d_used, pg->nslots, hipMemcpyDeviceToHost));
+    if (!use_acc.empty()) {
+      // trace: each package used this round counts once
+      std::vector<uint8_t> seen(np, 0);
+      for (uint64_t sl = 0; sl < pg->nslots; sl++) {
+        const int32_t own = pg->policy.owner((uint32_t)sl);
+        if (used[sl] && own >= 0 && !seen[own]) {
+          seen[own] = 1;
+          use_acc[own]++;
+        }
+      }
+    }
+    if (npend == 0 && !edges_left && !tasks_left) break;
+    pg->policy.touched(used.data(), (uint32_t)pg->nslots);
+    // admission: the oldest pieces of the window while their needs fit
+    const uint64_t nw = std::min(npend, kWindow);
+    if (nw) {
+      hipLaunchKernelGGL(k_piece_needs, dim3((unsigned)std::min<uint64_t>((nw + 255) / 256, 1024)),
+                         dim3(256), 0, st, (const Resume*)g_ws.pend[pc], nw, pg->d_pkg_of,
+                         (uint32_t*)g_ws.d_needs);
+      HIP_OK(hipMemcpyAsync(g_ws.h_needs, g_ws.d_needs, sizeof(uint32_t) * nw * kNeed,
+                            hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+    }
+    std::vector<uint8_t> in_set(np, 0);
+    std::vector<uint32_t> want;
+    uint64_t set_slots = 0;
+    size_t nadmit = 0;
+    for (uint64_t i = 0; i < nw; i++) {
+      const uint32_t* need = &g_ws.h_needs[i * kNeed + 1];
+      const int nneed = (int)g_ws.h_needs[i * kNeed];
+      uint64_t extra = 0;
+      for (int q = 0; q < nneed; q++) {
+        bool dup = in_set[need[q]] != 0;
+        for (int z = 0; z < q && !dup; z++) dup = need[z] == need[q];
+        if (!dup) extra += PL.nslots[need[q]];
+      }
+      if (set_slots + extra > pg->nslots) break;
+      for (int q = 0; q < nneed; q++)
+        if (!in_set[need[q]]) {
+          in_set[need[q]] = 1;
+          want.push_back(need[q]);
+        }
+      set_slots += extra;
+      nadmit++;
+    }
+    if (nadmit == 0 && npend && !edges_left && !tasks_left) {
+      set_error("out-of-core: a deferred search needs more packages than the cache holds");
+      return PM_ERR_OVERFLOW;
+    }
+    std::vector<uint8_t> pinned(pg->nslots, 0);
+    for (uint32_t q : want)
+      if (pg->pkg_slot[q] >= 0)
+        for (uint32_t w = 0; w < PL.nslots[q]; w++) pinned[pg->pkg_slot[q] + w] = 1;
+    const auto c0 = std::chrono::steady_clock::now();
+    uint64_t b = 0;
+    rc = load_packages(pg, want, pinned, g_ws.copy_stream, &b);
+    if (rc) return rc;
+    rc = flush_packages(pg, g_ws.copy_stream);
+    if (rc) return rc;
+    rs.link_bytes += b;
+    if (opts->prefetch && !by_score.empty()) {
+      // learned prefetch: free slots take the packages of highest
+      // predicted use that are not resident (nothing is evicted for it)
+      uint64_t free_slots = 0;
+      std::vector<uint8_t> keep(pg->nslots, 1);
+      for (uint64_t sl = 0; sl < pg->nslots; sl++)
+        if (pg->policy.owner((uint32_t)sl) < 0) {
+          free_slots++;
+          keep[sl] = 0;
+        }
+      std::vector<uint32_t> pre;
+      for (uint32_t q : by_score) {
+        if (free_slots == 0) break;
+        if (pg->pkg_slot[q] >= 0 || PL.nslots[q] > free_slots) continue;
+        pre.push_back(q);
+        free_slots -= PL.nslots[q];
+      }
+      if (!pre.empty()) {
+        rc = load_packages(pg, pre, keep, g_ws.copy_stream, &b);
+        if (rc) return rc;
+        rc = flush_packages(pg, g_ws.copy_stream);
+        if (rc) return rc;
+        rs.link_bytes += b;
+      }
+    }
+    HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
+                          hipMemcpyHostToDevice, g_ws.copy_stream));
+    HIP_OK(hipStreamSynchronize(g_ws.copy_stream));
+    rs.link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
+                   .count();
+    // every pending piece whose needs are now resident runs this round
+    uint64_t nrun = 0;
+    if (npend) {
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor + 6, 2, 0ull);
+      hipLaunchKernelGGL(k_piece_select,
+                         dim3((unsigned)std::min<uint64_t>((npend + 255) / 256, 16384)),
+                         dim3(256), 0, st, (const Resume*)g_ws.pend[pc], npend, pg->d_pkg_of,
+                         pg->d_pkg_slot, (Resume*)g_ws.resume[0], (Resume*)g_ws.pend[1 - pc],
+                         g_ws.cursor + 6);
+      unsigned long long sel[2];
+      HIP_OK(hipMemcpyAsync(sel, g_ws.cursor + 6, sizeof(sel), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      nrun = sel[0];
+      npend = sel[1];
+      pc = 1 - pc;
+    }
+    // next round: continue the edge scan and the task queue, resume the
+    // selected pieces (they may split and defer again)
+    unsigned long long nc[5] = {cur8[0], tasks_left ? tasks_done : 0ull,
+                                tasks_left ? cur8[2] : 0ull, 0ull, 0ull};
+    HIP_OK(hipMemcpyAsync(g_ws.cursor, nc, sizeof(nc), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_miss_bits,
+                       (uint64_t)((np + 31) / 32 + 1), 0u);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, pg->d_miss_count, 1, 0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_used,
+                       (pg->nslots + 3) / 4, 0u);
+    L.rin = (const Resume*)g_ws.resume[0];
+    L.nrin = nrun;
+    L.rout = (Resume*)g_ws.resume[1];
+    const auto k0 = std::chrono::steady_clock::now();
+    if (edges_left) hipLaunchKernelGGL(k_match_edges<true>, dim3(waves), dim3(kWave), 0, st, L);
+    if (L.nrin) hipLaunchKernelGGL(k_resume<true>, dim3(waves), dim3(kWave), 0, st, L);
+    // tasks left over, or written this round by the edge scan or by
+    // resumed depth-0 pieces
+    if (L.tasks && (tasks_left || L.nrin || edges_left))
+      hipLaunchKernelGGL(k_match_tasks<true>, dim3(waves), dim3(kWave), 0, st, L);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+    rs.kernel_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - k0).count();
+    rs.rounds++;
+    if (rs.rounds > 1000000) {
+      set_error("out-of-core: no progress");
+      return PM_ERR_OVERFLOW;
+    }
+  }
+  L.rin = nullptr;
+  L.nrin = 0;
+  return PM_OK;
+}
+
 int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res) {
   const Plan* one = &plan;
   return device_match_batch(g, &one, 1, opts, res);
@@ -2457,203 +2668,15 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     phase[1] = me;
     phase[2] = mt;
     if (paged) {
-      // Rounds.  Deferred searches wait on the host; each round admits them
-      // oldest first while the union of what they need -- the packages of
-      // their path rows and of the package that stopped them -- fits the
-      // cache, loads exactly that set (pinned against eviction) and resumes
-      // only those.  Every admitted search then passes its stopping point,
-      // so the rounds terminate; nothing is dropped, so counts are exact.
-      const PackageLayout& PL = pg->layout;
-      const uint64_t span = (uint64_t)npat * nchunk;  // phase-1 units
-      // Pending pieces stay in HBM (pend[pc][0..npend)); each round the
-      // host reads the needs of the first kWindow of them, admits those
-      // oldest first while the union of their needs fits the cache, loads
-      // that set, and the device moves every pending piece whose needs are
-      // resident to the resume buffer.
-      constexpr uint64_t kWindow = 1u << 16;
-      constexpr uint64_t kPendMax = 1ull << 26;  // pieces held between rounds
-      uint64_t npend = 0;
-      int pc = 0;
-      rc = grow_pinned_u32(&g_ws.h_needs, &g_ws.h_needs_cap, kWindow * kNeed);
+      RoundStats rs;
+      rc = run_rounds(pg, L, st, waves, (uint64_t)npat * nchunk, rcap, opts, by_score, use_acc, rs);
       if (rc) return rc;
-      rc = grow(&g_ws.d_needs, &g_ws.d_needs_bytes, sizeof(uint32_t) * kWindow * kNeed);
-      if (rc) return rc;
-      for (;;) {
-        unsigned long long cur8[8], mc = 0;
-        HIP_OK(hipMemcpy(cur8, g_ws.cursor, sizeof(cur8), hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(&mc, pg->d_miss_count, sizeof(mc), hipMemcpyDeviceToHost));
-        const uint64_t nd = cur8[4];
-        if (nd > rcap) {
-          set_error("out-of-core: more deferred pieces in one round than the buffer holds");
-          return PM_ERR_OVERFLOW;
-        }
-        if (cur8[2] > g_ws.task_cap) {
-          set_error("out-of-core: range task buffer overflow");
-          return PM_ERR_OVERFLOW;
-        }
-        misses += mc;
-        deferred += nd;
-        if (nd) {
-          if (npend + nd > kPendMax) {
-            set_error("out-of-core: too many deferred pieces pending");
-            return PM_ERR_OVERFLOW;
-          }
-          // pending buffers hold npend + nd; the run buffer may take all
-          for (int i = 0; i < 2; i++) {
-            rc = grow_keep(&g_ws.pend[i], &g_ws.pend_bytes[i], sizeof(Resume) * (npend + nd),
-                           i == pc ? sizeof(Resume) * npend : 0, st);
-            if (rc) return rc;
-          }
-          rc = grow_keep(&g_ws.resume[0], &g_ws.resume_bytes[0], sizeof(Resume) * (npend + nd), 0,
-                         st);
-          if (rc) return rc;
-          HIP_OK(hipMemcpyAsync((Resume*)g_ws.pend[pc] + npend, g_ws.resume[1],
-                                sizeof(Resume) * nd, hipMemcpyDeviceToDevice, st));
-          npend += nd;
-        }
-        const bool edges_left = cur8[0] < span;
-        const uint64_t tasks_done = std::min(cur8[1], cur8[2]);
-        const bool tasks_left = L.tasks && tasks_done < cur8[2];
-        std::vector<uint8_t> used(pg->nslots);
-        HIP_OK(hipMemcpy(used.data(), pg->d_used, pg->nslots, hipMemcpyDeviceToHost));
-        if (!use_acc.empty()) {
-          // trace: each package used this round counts once
-          std::vector<uint8_t> seen(np, 0);
-          for (uint64_t sl = 0; sl < pg->nslots; sl++) {
-            const int32_t own = pg->policy.owner((uint32_t)sl);
-            if (used[sl] && own >= 0 && !seen[own]) {
-              seen[own] = 1;
-              use_acc[own]++;
-            }
-          }
-        }
-        if (npend == 0 && !edges_left && !tasks_left) break;
-        pg->policy.touched(used.data(), (uint32_t)pg->nslots);
-        // admission: the oldest pieces of the window while their needs fit
-        const uint64_t nw = std::min(npend, kWindow);
-        if (nw) {
-          hipLaunchKernelGGL(k_piece_needs, dim3((unsigned)std::min<uint64_t>((nw + 255) / 256, 1024)),
-                             dim3(256), 0, st, (const Resume*)g_ws.pend[pc], nw, pg->d_pkg_of,
-                             (uint32_t*)g_ws.d_needs);
-          HIP_OK(hipMemcpyAsync(g_ws.h_needs, g_ws.d_needs, sizeof(uint32_t) * nw * kNeed,
-                                hipMemcpyDeviceToHost, st));
-          HIP_OK(hipStreamSynchronize(st));
-        }
-        std::vector<uint8_t> in_set(np, 0);
-        std::vector<uint32_t> want;
-        uint64_t set_slots = 0;
-        size_t nadmit = 0;
-        for (uint64_t i = 0; i < nw; i++) {
-          const uint32_t* need = &g_ws.h_needs[i * kNeed + 1];
-          const int nneed = (int)g_ws.h_needs[i * kNeed];
-          uint64_t extra = 0;
-          for (int q = 0; q < nneed; q++) {
-            bool dup = in_set[need[q]] != 0;
-            for (int z = 0; z < q && !dup; z++) dup = need[z] == need[q];
-            if (!dup) extra += PL.nslots[need[q]];
-          }
-          if (set_slots + extra > pg->nslots) break;
-          for (int q = 0; q < nneed; q++)
-            if (!in_set[need[q]]) {
-              in_set[need[q]] = 1;
-              want.push_back(need[q]);
-            }
-          set_slots += extra;
-          nadmit++;
-        }
-        if (nadmit == 0 && npend && !edges_left && !tasks_left) {
-          set_error("out-of-core: a deferred search needs more packages than the cache holds");
-          return PM_ERR_OVERFLOW;
-        }
-        std::vector<uint8_t> pinned(pg->nslots, 0);
-        for (uint32_t q : want)
-          if (pg->pkg_slot[q] >= 0)
-            for (uint32_t w = 0; w < PL.nslots[q]; w++) pinned[pg->pkg_slot[q] + w] = 1;
-        const auto c0 = std::chrono::steady_clock::now();
-        uint64_t b = 0;
-        rc = load_packages(pg, want, pinned, g_ws.copy_stream, &b);
-        if (rc) return rc;
-        rc = flush_packages(pg, g_ws.copy_stream);
-        if (rc) return rc;
-        link_bytes += b;
-        if (opts->prefetch && !by_score.empty()) {
-          // learned prefetch: free slots take the packages of highest
-          // predicted use that are not resident (nothing is evicted for it)
-          uint64_t free_slots = 0;
-          std::vector<uint8_t> keep(pg->nslots, 1);
-          for (uint64_t sl = 0; sl < pg->nslots; sl++)
-            if (pg->policy.owner((uint32_t)sl) < 0) {
-              free_slots++;
-              keep[sl] = 0;
-            }
-          std::vector<uint32_t> pre;
-          for (uint32_t q : by_score) {
-            if (free_slots == 0) break;
-            if (pg->pkg_slot[q] >= 0 || PL.nslots[q] > free_slots) continue;
-            pre.push_back(q);
-            free_slots -= PL.nslots[q];
-          }
-          if (!pre.empty()) {
-            rc = load_packages(pg, pre, keep, g_ws.copy_stream, &b);
-            if (rc) return rc;
-            rc = flush_packages(pg, g_ws.copy_stream);
-            if (rc) return rc;
-            link_bytes += b;
-          }
-        }
-        HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
-                              hipMemcpyHostToDevice, g_ws.copy_stream));
-        HIP_OK(hipStreamSynchronize(g_ws.copy_stream));
-        link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
-                       .count();
-        // every pending piece whose needs are now resident runs this round
-        uint64_t nrun = 0;
-        if (npend) {
-          hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor + 6, 2, 0ull);
-          hipLaunchKernelGGL(k_piece_select,
-                             dim3((unsigned)std::min<uint64_t>((npend + 255) / 256, 16384)),
-                             dim3(256), 0, st, (const Resume*)g_ws.pend[pc], npend, pg->d_pkg_of,
-                             pg->d_pkg_slot, (Resume*)g_ws.resume[0], (Resume*)g_ws.pend[1 - pc],
-                             g_ws.cursor + 6);
-          unsigned long long sel[2];
-          HIP_OK(hipMemcpyAsync(sel, g_ws.cursor + 6, sizeof(sel), hipMemcpyDeviceToHost, st));
-          HIP_OK(hipStreamSynchronize(st));
-          nrun = sel[0];
-          npend = sel[1];
-          pc = 1 - pc;
-        }
-        // next round: continue the edge scan and the task queue, resume the
-        // selected pieces (they may split and defer again)
-        unsigned long long nc[5] = {cur8[0], tasks_left ? tasks_done : 0ull,
-                                    tasks_left ? cur8[2] : 0ull, 0ull, 0ull};
-        HIP_OK(hipMemcpyAsync(g_ws.cursor, nc, sizeof(nc), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_miss_bits,
-                           (uint64_t)((np + 31) / 32 + 1), 0u);
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, pg->d_miss_count, 1, 0ull);
-        hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_used,
-                           (pg->nslots + 3) / 4, 0u);
-        L.rin = (const Resume*)g_ws.resume[0];
-        L.nrin = nrun;
-        L.rout = (Resume*)g_ws.resume[1];
-        const auto k0 = std::chrono::steady_clock::now();
-        if (edges_left) hipLaunchKernelGGL(k_match_edges<true>, dim3(waves), dim3(kWave), 0, st, L);
-        if (L.nrin) hipLaunchKernelGGL(k_resume<true>, dim3(waves), dim3(kWave), 0, st, L);
-        // tasks left over, or written this round by the edge scan or by
-        // resumed depth-0 pieces
-        if (L.tasks && (tasks_left || L.nrin || edges_left))
-          hipLaunchKernelGGL(k_match_tasks<true>, dim3(waves), dim3(kWave), 0, st, L);
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipStreamSynchronize(st));
-        round_kernel_ms +=
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - k0).count();
-        rounds++;
-        if (rounds > 1000000) {
-          set_error("out-of-core: no progress");
-          return PM_ERR_OVERFLOW;
-        }
-      }
-      L.rin = nullptr;
-      L.nrin = 0;
+      misses = rs.misses;
+      deferred = rs.deferred;
+      link_bytes += rs.link_bytes;
+      link_ms += rs.link_ms;
+      round_kernel_ms += rs.kernel_ms;
+      rounds = rs.rounds;
     }
     const double wall =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
