@@ -84,6 +84,7 @@ constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
 #define PM_ISECT_DUAL 0
 #endif
 constexpr int kWalk = PM_FUSED_WALK;
+constexpr uint32_t kResumeClaim = 16;  // deferred pieces claimed per atomic
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kMaxDepth = 6;
 
@@ -188,6 +189,11 @@ struct WaveState {
   uint32_t jr[kWave];
   uint32_t jc[kWave];
   unsigned long long jedge[kWave];  // CSR edge of each lane's job
+  // resumed group of one-candidate pieces on one search path: their
+  // candidate indices at the fused level and the vertices there
+  uint32_t lcount;                  // 0: not a group
+  uint32_t lidx[kWave];
+  uint32_t ltmp[kWave];
   unsigned long long alg0;          // work counter at the start of the job (trace)
   // fused step: per-owner data of the current batch
   uint32_t own_off[kWave + 1];
@@ -689,7 +695,9 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
   const unsigned long long wd = uni64(ws.wacc[d]);
   const unsigned long long* w1 = P.wt[u1];
   const unsigned long long* w2 = P.wt[u2];
-  const uint32_t* cand = uniptr(ws.sptr[P.cand_slot[d]]);
+  // a resumed group walks its listed candidates (ltmp, indices lidx)
+  const bool listed = uni(ws.lcount) != 0;
+  const uint32_t* cand = listed ? (const uint32_t*)ws.ltmp : uniptr(ws.sptr[P.cand_slot[d]]);
   const uint32_t cbeg = uni(ws.cnext[d]), cend = uni(ws.clen[d]);
   // u2's bound from vertices 0..d, and whether phi(u2) < phi(u1)
   const uint32_t ub_fixed = lab ? kInf : vertex_bound(P, ws, u2, d);
@@ -756,7 +764,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
       uint32_t rest_from = 0;
       const uint64_t mm = __ballot(miss);
       if (mm) {
-        if (uni64(*(volatile unsigned long long*)O.count) > O.soft) {
+        if (!listed && uni64(*(volatile unsigned long long*)O.count) > O.soft) {
           // too many pieces already: finish the owners before the first
           // miss; the rest of the range becomes one piece
           const int f = __builtin_ctzll(mm);
@@ -766,7 +774,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
           if ((int)lane >= f) ok = false;
         } else {
           // owners whose row is not resident become deferred pieces
-          emit_lanes(O, ws, d, miss, i, mp, lane);
+          emit_lanes(O, ws, d, miss, listed && i < cend ? ws.lidx[i] : i, mp, lane);
           ok = ok && !miss;
         }
       }
@@ -839,7 +847,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
     uint32_t rest_from = 0;
     const uint64_t mm = __ballot(miss);
     if (mm) {
-      if (uni64(*(volatile unsigned long long*)O.count) > O.soft) {
+      if (!listed && uni64(*(volatile unsigned long long*)O.count) > O.soft) {
         // too many pieces already: finish the owners before the first miss;
         // the rest of the range becomes one piece
         const int f = __builtin_ctzll(mm);
@@ -849,7 +857,7 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
         if ((int)lane >= f) ok = false;
       } else {
         // owners whose row is not resident become deferred pieces
-        emit_lanes(O, ws, d, miss, i, mp, lane);
+        emit_lanes(O, ws, d, miss, listed && i < cend ? ws.lidx[i] : i, mp, lane);
         ok = ok && !miss;
       }
     }
@@ -1112,13 +1120,28 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
       ws.phi[dd + 1] = x;
       ws.wacc[dd + 1] = uni64(ws.wacc[dd]) * wt_of(P, dd + 1, x);
       if (!run_ops<PG>(g, P, ws, dd + 1, arena, cap, lane, ct)) {
-        emit_one(O, ws, D, uni(ws.ridx[D]), uni(ws.rend), uni(ws.blk), lane);
+        const uint32_t nl = uni(ws.lcount);
+        if (nl == 0) emit_one(O, ws, D, uni(ws.ridx[D]), uni(ws.rend), uni(ws.blk), lane);
+        for (uint32_t q = 0; q < nl; q++) {
+          const uint32_t iq = uni(ws.lidx[q]);
+          emit_one(O, ws, D, iq, iq + 1, uni(ws.blk), lane);
+        }
         return 0;
       }
       setup_cand(P, ws, dd + 1);
     }
-    ws.cnext[D] = uni(ws.ridx[D]);
-    ws.clen[D] = min(uni(ws.clen[D]), uni(ws.rend));
+    const uint32_t nl = uni(ws.lcount);
+    if (nl) {
+      // a group of one-candidate pieces: their vertices in index order
+      const uint32_t* cs = uniptr(ws.sptr[P.cand_slot[D]]);
+      if (lane < nl) ws.ltmp[lane] = cs[ws.lidx[lane]];
+      __syncthreads();
+      ws.cnext[D] = 0;
+      ws.clen[D] = nl;
+    } else {
+      ws.cnext[D] = uni(ws.ridx[D]);
+      ws.clen[D] = min(uni(ws.clen[D]), uni(ws.rend));
+    }
     ws.cmask[D] = 0;
     d = D;
     stop = D;
@@ -1274,6 +1297,7 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_edges(
   if (lane == 0) {
     ws.pat = 0xFFFFFFFFu;
     ws.acc_count = ws.acc_jobs = 0;
+    ws.lcount = 0;
   }
   __syncthreads();
   const DeferOut O = defer_out(L);
@@ -1369,6 +1393,8 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_tasks(
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
   Counters ct;
   const DeferOut O = defer_out(L);
+  if (lane == 0) ws.lcount = 0;
+  __syncthreads();
   const uint64_t ntask = L.tasks ? min((uint64_t)L.cursor[2], L.task_cap) : 0;
   for (;;) {
     // out of core: likewise for tasks (claims then stay below the count)
@@ -1409,29 +1435,61 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_resume(Launc
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
   Counters ct;
   const DeferOut O = defer_out(L);
+  if (lane == 0) ws.lcount = 0;
+  __syncthreads();
+  // Pieces are claimed kResumeClaim at a time.  Consecutive one-candidate
+  // pieces at the fused level with the same search path (the ones a single
+  // batch of candidates deferred) run as one group: the path is rebuilt
+  // once and the fused step walks all their candidates together.
   for (;;) {
-    unsigned long long t = 0;
-    if (lane == 0) t = atomicAdd(&L.cursor[3], 1ull);
-    t = uni64(__shfl(t, 0));
-    if (t >= L.nrin) break;
-    const uint64_t e = uni64(L.rin[t].edge);
-    const uint32_t r = uni(L.rin[t].r), c = uni(L.rin[t].c);
-    const uint32_t rlo = uni(L.rin[t].lo), rhi = uni(L.rin[t].hi);
-    const int depth = (int)uni((uint32_t)L.rin[t].depth);
-    const uint32_t p = uni(L.rin[t].pat);
-    ws.pat = p;
-    ws.rdepth = depth;
-    for (int d = 1; d <= depth && d < kMaxDepth; d++) ws.ridx[d] = uni(L.rin[t].idx[d]);
-    for (int d = 0; d <= depth && d < kMaxDepth; d++) ws.phi[d] = uni(L.rin[t].phi[d]);
-    ws.rend = uni(L.rin[t].end);
-    if (lane == 0) ws.alg0 = ct.alg;
-    const unsigned long long cnt =
-        run_job<PG>(g, L.plans[p], ws, arena, L.cap, lane, r, c, e, rlo, rhi, true,
-                    depth == 0 && L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
-    if (L.root_work && lane == 0)
-      atomicAdd(&L.root_work[r], (unsigned long long)(ct.alg - ws.alg0));
-    if (lane == 0 && cnt) atomicAdd(&L.pcount[2 * p], cnt);
-    if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
+    unsigned long long t0 = 0;
+    if (lane == 0) t0 = atomicAdd(&L.cursor[3], (unsigned long long)kResumeClaim);
+    t0 = uni64(__shfl(t0, 0));
+    if (t0 >= L.nrin) break;
+    const uint64_t t1 = min(t0 + kResumeClaim, L.nrin);
+    for (uint64_t t = t0; t < t1;) {
+      const Resume& R0 = L.rin[t];
+      const uint64_t e = uni64(R0.edge);
+      const uint32_t r = uni(R0.r), c = uni(R0.c);
+      const uint32_t rlo = uni(R0.lo), rhi = uni(R0.hi);
+      const int depth = (int)uni((uint32_t)R0.depth);
+      const uint32_t p = uni(R0.pat);
+      const KPlan& P = L.plans[p];
+      uint64_t t2 = t + 1;
+      if (depth >= 1 && depth < kMaxDepth && depth == P.k - 3 &&
+          uni(R0.end) == uni(R0.idx[depth]) + 1) {
+        while (t2 < t1 && t2 - t < (uint64_t)kWave) {
+          const Resume& R = L.rin[t2];
+          bool same = uni64(R.edge) == e && uni(R.pat) == p && uni((uint32_t)R.depth) == (uint32_t)depth &&
+                      uni(R.lo) == rlo && uni(R.hi) == rhi && uni(R.end) == uni(R.idx[depth]) + 1;
+          for (int d = 1; d < depth && same; d++) same = uni(R.idx[d]) == uni(R0.idx[d]);
+          if (!same) break;
+          t2++;
+        }
+      }
+      ws.pat = p;
+      ws.rdepth = depth;
+      for (int d = 1; d <= depth && d < kMaxDepth; d++) ws.ridx[d] = uni(R0.idx[d]);
+      for (int d = 0; d <= depth && d < kMaxDepth; d++) ws.phi[d] = uni(R0.phi[d]);
+      ws.rend = uni(R0.end);
+      const uint32_t ng = (uint32_t)(t2 - t);
+      if (ng > 1 && lane < ng) ws.lidx[lane] = L.rin[t + lane].idx[depth];
+      if (lane == 0) {
+        ws.lcount = ng > 1 ? ng : 0;
+        ws.alg0 = ct.alg;
+      }
+      __syncthreads();
+      const unsigned long long cnt =
+          run_job<PG>(g, P, ws, arena, L.cap, lane, r, c, e, rlo, rhi, true,
+                      depth == 0 && L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
+      if (lane == 0) ws.lcount = 0;
+      __syncthreads();
+      if (L.root_work && lane == 0)
+        atomicAdd(&L.root_work[r], (unsigned long long)(ct.alg - ws.alg0));
+      if (lane == 0 && cnt) atomicAdd(&L.pcount[2 * p], cnt);
+      if (L.per_edge && lane == 0 && cnt) atomicAdd(&L.per_edge[e - L.edge_lo], cnt);
+      t = t2;
+    }
   }
   if (lane == 0) {
     atomicAdd(&L.out[2], (unsigned long long)ct.alg);
