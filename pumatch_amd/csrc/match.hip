@@ -1260,6 +1260,7 @@ struct Launch {
   uint64_t task_cap;
   const Resume* rin;  // deferred pieces resumed this round
   uint64_t nrin;
+  const unsigned long long* nrin_dev;  // when set, the count is read from here
   Resume* rout;  // deferred pieces for the next round
   uint64_t rout_cap;
   uint64_t soft;  // beyond this many new pieces, searches stop at their first miss
@@ -1437,6 +1438,7 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_resume(Launc
   const DeferOut O = defer_out(L);
   if (lane == 0) ws.lcount = 0;
   __syncthreads();
+  const uint64_t nrin = L.nrin_dev ? uni64(*L.nrin_dev) : L.nrin;
   // Pieces are claimed kResumeClaim at a time.  Consecutive one-candidate
   // pieces at the fused level with the same search path (the ones a single
   // batch of candidates deferred) run as one group: the path is rebuilt
@@ -1445,8 +1447,8 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_resume(Launc
     unsigned long long t0 = 0;
     if (lane == 0) t0 = atomicAdd(&L.cursor[3], (unsigned long long)kResumeClaim);
     t0 = uni64(__shfl(t0, 0));
-    if (t0 >= L.nrin) break;
-    const uint64_t t1 = min(t0 + kResumeClaim, L.nrin);
+    if (t0 >= nrin) break;
+    const uint64_t t1 = min(t0 + kResumeClaim, nrin);
     for (uint64_t t = t0; t < t1;) {
       const Resume& R0 = L.rin[t];
       const uint64_t e = uni64(R0.edge);
@@ -1713,6 +1715,9 @@ struct Workspace {
   size_t d_needs_bytes = 0;
   uint32_t* h_needs = nullptr;         // pinned copy of the same
   size_t h_needs_cap = 0;
+  unsigned long long* h_stat = nullptr;  // pinned round status
+  uint8_t* h_used = nullptr;             // pinned per-slot use of a round
+  size_t h_used_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, eve = nullptr, evc = nullptr;
   int cus = 0;
   int resident = 0;
@@ -2113,7 +2118,7 @@ int load_packages(PagedGraph* pg, const std::vector<uint32_t>& want, std::vector
 }
 
 // Issue the package copies gathered by load_packages (one launch on cs).
-int flush_packages(PagedGraph* pg, hipStream_t cs) {
+int flush_packages(PagedGraph* pg, hipStream_t cs, bool wait = true) {
   if (pg->jobs.empty()) return PM_OK;
   HIP_OK(hipMemcpyAsync(pg->d_jobs, pg->jobs.data(), sizeof(CopyJob) * pg->jobs.size(),
                         hipMemcpyHostToDevice, cs));
@@ -2121,7 +2126,9 @@ int flush_packages(PagedGraph* pg, hipStream_t cs) {
   hipLaunchKernelGGL(k_copy_packages, dim3(std::min<uint32_t>((nj + 3) / 4, 4096)), dim3(256), 0, cs,
                      pg->host_rows_dev, pg->d_cache, pg->d_jobs, nj);
   HIP_OK(hipGetLastError());
-  // the staging vector is reused after the stream has drained
+  // the staging vector is reused after the stream has drained (without
+  // `wait`, the caller waits and clears it)
+  if (!wait) return PM_OK;
   HIP_OK(hipStreamSynchronize(cs));
   pg->jobs.clear();
   return PM_OK;
@@ -2195,10 +2202,35 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
   if (rc) return rc;
   rc = grow(&g_ws.d_needs, &g_ws.d_needs_bytes, sizeof(uint32_t) * kWindow * kNeed);
   if (rc) return rc;
+  // Round status (cursors, miss count, selection counts, slot use) comes
+  // back in one batch of copies into pinned memory after each round's
+  // kernels: [0..7] cursors, [8] misses.
+  if (!g_ws.h_stat) HIP_OK(hipHostMalloc((void**)&g_ws.h_stat, sizeof(unsigned long long) * 16,
+                                         hipHostMallocDefault));
+  if (g_ws.h_used_cap < pg->nslots) {
+    if (g_ws.h_used) HIP_OK(hipHostFree(g_ws.h_used));
+    g_ws.h_used = nullptr;
+    g_ws.h_used_cap = 0;
+    HIP_OK(hipHostMalloc((void**)&g_ws.h_used, pg->nslots, hipHostMallocDefault));
+    g_ws.h_used_cap = pg->nslots;
+  }
+  auto fetch_status = [&]() -> int {
+    HIP_OK(hipMemcpyAsync(g_ws.h_stat, g_ws.cursor, sizeof(unsigned long long) * 8,
+                          hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(g_ws.h_stat + 8, pg->d_miss_count, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(g_ws.h_used, pg->d_used, pg->nslots, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return PM_OK;
+  };
+  rc = fetch_status();
+  if (rc) return rc;
+  bool selected = false;  // the last round ran the selection (its keep count is pending)
   for (;;) {
-    unsigned long long cur8[8], mc = 0;
-    HIP_OK(hipMemcpy(cur8, g_ws.cursor, sizeof(cur8), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(&mc, pg->d_miss_count, sizeof(mc), hipMemcpyDeviceToHost));
+    unsigned long long cur8[8];
+    std::memcpy(cur8, g_ws.h_stat, sizeof(cur8));
+    const unsigned long long mc = g_ws.h_stat[8];
+    if (selected) npend = cur8[7];  // pieces the selection kept
     const uint64_t nd = cur8[4];
     if (nd > rcap) {
       set_error("out-of-core: more deferred pieces in one round than the buffer holds");
@@ -2231,8 +2263,7 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
     const bool edges_left = cur8[0] < span;
     const uint64_t tasks_done = std::min(cur8[1], cur8[2]);
     const bool tasks_left = L.tasks && tasks_done < cur8[2];
-    std::vector<uint8_t> used(pg->nslots);
-    HIP_OK(hipMemcpy(used.data(), pg->d_used, pg->nslots, hipMemcpyDeviceToHost));
+    const uint8_t* used = g_ws.h_used;
     if (!use_acc.empty()) {
       // trace: each package used this round counts once
       std::vector<uint8_t> seen(np, 0);
@@ -2245,7 +2276,7 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
       }
     }
     if (npend == 0 && !edges_left && !tasks_left) break;
-    pg->policy.touched(used.data(), (uint32_t)pg->nslots);
+    pg->policy.touched(used, (uint32_t)pg->nslots);
     // admission: the oldest pieces of the window while their needs fit
     const uint64_t nw = std::min(npend, kWindow);
     if (nw) {
@@ -2290,8 +2321,6 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
     uint64_t b = 0;
     rc = load_packages(pg, want, pinned, g_ws.copy_stream, &b);
     if (rc) return rc;
-    rc = flush_packages(pg, g_ws.copy_stream);
-    if (rc) return rc;
     rs.link_bytes += b;
     if (opts->prefetch && !by_score.empty()) {
       // learned prefetch: free slots take the packages of highest
@@ -2313,30 +2342,30 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
       if (!pre.empty()) {
         rc = load_packages(pg, pre, keep, g_ws.copy_stream, &b);
         if (rc) return rc;
-        rc = flush_packages(pg, g_ws.copy_stream);
-        if (rc) return rc;
         rs.link_bytes += b;
       }
     }
+    // one copy launch for the round's loads, the new slot map, one wait
+    rc = flush_packages(pg, g_ws.copy_stream, false);
+    if (rc) return rc;
     HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
                           hipMemcpyHostToDevice, g_ws.copy_stream));
     HIP_OK(hipStreamSynchronize(g_ws.copy_stream));
+    pg->jobs.clear();
     rs.link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
                    .count();
-    // every pending piece whose needs are now resident runs this round
-    uint64_t nrun = 0;
-    if (npend) {
+    // every pending piece whose needs are now resident runs this round;
+    // the run count stays on the device (cursor[6]) for k_resume, the kept
+    // count (cursor[7]) comes back with the next status
+    const bool run_pieces = npend > 0;
+    selected = run_pieces;
+    if (run_pieces) {
       hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor + 6, 2, 0ull);
       hipLaunchKernelGGL(k_piece_select,
                          dim3((unsigned)std::min<uint64_t>((npend + 255) / 256, 16384)),
                          dim3(256), 0, st, (const Resume*)g_ws.pend[pc], npend, pg->d_pkg_of,
                          pg->d_pkg_slot, (Resume*)g_ws.resume[0], (Resume*)g_ws.pend[1 - pc],
                          g_ws.cursor + 6);
-      unsigned long long sel[2];
-      HIP_OK(hipMemcpyAsync(sel, g_ws.cursor + 6, sizeof(sel), hipMemcpyDeviceToHost, st));
-      HIP_OK(hipStreamSynchronize(st));
-      nrun = sel[0];
-      npend = sel[1];
       pc = 1 - pc;
     }
     // next round: continue the edge scan and the task queue, resume the
@@ -2350,17 +2379,19 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, st, (uint32_t*)pg->d_used,
                        (pg->nslots + 3) / 4, 0u);
     L.rin = (const Resume*)g_ws.resume[0];
-    L.nrin = nrun;
+    L.nrin = run_pieces ? ~0ull : 0;  // the count is read from cursor[6]
+    L.nrin_dev = run_pieces ? g_ws.cursor + 6 : nullptr;
     L.rout = (Resume*)g_ws.resume[1];
     const auto k0 = std::chrono::steady_clock::now();
     if (edges_left) hipLaunchKernelGGL(k_match_edges<true>, dim3(waves), dim3(kWave), 0, st, L);
     if (L.nrin) hipLaunchKernelGGL(k_resume<true>, dim3(waves), dim3(kWave), 0, st, L);
     // tasks left over, or written this round by the edge scan or by
     // resumed depth-0 pieces
-    if (L.tasks && (tasks_left || L.nrin || edges_left))
+    if (L.tasks && (tasks_left || run_pieces || edges_left))
       hipLaunchKernelGGL(k_match_tasks<true>, dim3(waves), dim3(kWave), 0, st, L);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipStreamSynchronize(st));
+    rc = fetch_status();  // waits for the round's kernels
+    if (rc) return rc;
     rs.kernel_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - k0).count();
     rs.rounds++;
