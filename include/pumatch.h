@@ -158,10 +158,18 @@ void pm_graph_destroy(pm_graph* g);
 
 /* -- patterns ------------------------------------------------------------- */
 
-/* Parse a gSpan-like .g file and plan it.  labeled=1 uses the vertex labels
- * of the file (LABELED=true, src/config.h:14); labeled=0 sets every label to
- * 1 and counts embeddings with symmetry breaking.  Replaces
- * PatternPreprocessor(path) (src/pattern.h:48-58). */
+/* Pattern modes (the `labeled` argument below is a set of these bits). */
+#define PM_LABELED 0x1        /* LABELED=true, src/config.h:14 */
+#define PM_VERTEX_INDUCED 0x2 /* EDGE_INDUCED=false, src/config.h:15: pattern
+                                 non-edges map to non-adjacent vertices
+                                 (src/gpu_match.cu:466-470,570-587,734-738);
+                                 rows must be resident in HBM */
+
+/* Parse a gSpan-like .g file and plan it.  PM_LABELED (1) uses the vertex
+ * labels of the file (LABELED=true, src/config.h:14); without it every
+ * label is 1 and embeddings are counted with symmetry breaking.  Adding
+ * PM_VERTEX_INDUCED counts vertex-induced matches instead of edge-induced
+ * ones.  Replaces PatternPreprocessor(path) (src/pattern.h:48-58). */
 int pm_pattern_load(const char* path, int32_t labeled, pm_pattern** out);
 /* Same from the text of a .g file. */
 int pm_pattern_parse(const char* text, int32_t labeled, pm_pattern** out);
@@ -176,6 +184,7 @@ typedef struct {
   int32_t label[PM_MAX_PATTERN];
   uint32_t sym_mask[PM_MAX_PATTERN];  /* phi(u) < phi(j) for bit j */
   uint32_t nbr_mask[PM_MAX_PATTERN];  /* earlier neighbours of u */
+  int32_t vertex_induced;             /* PM_VERTEX_INDUCED was given */
 } pm_pattern_info;
 
 int pm_pattern_info_get(const pm_pattern* p, pm_pattern_info* info);

@@ -88,8 +88,10 @@ def plan_info(p: OrPattern, labeled: bool):
 
 
 def count(rowptr, colidx, labels, p: OrPattern, labeled: bool = True, edge_lo: int = 0,
-          edge_hi: int = 0, threads: int = 0, per_edge: bool = False):
-    """Total count (x|Aut| when unlabeled), job count, optional per-edge counts."""
+          edge_hi: int = 0, threads: int = 0, per_edge: bool = False, induced: bool = False):
+    """Total count (x|Aut| when unlabeled), job count, optional per-edge counts.
+    induced=True counts vertex-induced matches (pattern non-edges map to
+    non-adjacent vertices)."""
     rowptr = np.ascontiguousarray(rowptr, dtype=np.uint64)
     colidx = np.ascontiguousarray(colidx, dtype=np.uint32)
     lab = None
@@ -101,7 +103,7 @@ def count(rowptr, colidx, labels, p: OrPattern, labeled: bool = True, edge_lo: i
     out, jobs = C.c_uint64(), C.c_uint64()
     rc = lib().or_count(n, m, rowptr.ctypes.data, colidx.ctypes.data if m else None,
                         None if lab is None else lab.ctypes.data, C.byref(p),
-                        1 if labeled else 0, edge_lo, edge_hi, threads,
+                        (1 if labeled else 0) | (2 if induced else 0), edge_lo, edge_hi, threads,
                         None if pe is None else pe.ctypes.data, C.byref(out), C.byref(jobs))
     if rc:
         raise RuntimeError(f"oracle failed ({rc})")

@@ -31,6 +31,10 @@
  *             symmetry breaking, no multiplicity).
  *   unlabeled |Aut(P)| * sum over J of #{injective phi as above without labels,
  *             with phi(u) < phi(j) for every symmetry pair (j,u)}.
+ *   vertex-induced (EDGE_INDUCED=false, src/gpu_match.cu:466-470,570-587,
+ *             734-738): the same maps, and in addition phi(u) is not in
+ *             N(phi(j)) for every pattern non-edge j<u (the reference's
+ *             DIFF passes against the rows of the earlier non-neighbours).
  *
  * The reference's 4096-entry slot cap (src/gpu_match.cu:302,316) is a defect
  * and is not reproduced.
@@ -59,6 +63,7 @@ typedef struct {
   uint32_t X[OR_MAXK];    /* earlier non-neighbours of u (injectivity) */
   uint32_t lbit[OR_MAXK]; /* 1 << l_P(u) */
   int rc_order;           /* unlabeled job filter r > c, src/job_queue.h:44 */
+  int induced;            /* vertex-induced: non-edges map to non-edges */
 } or_plan;
 
 /* ---------------------------------------------------------------------- */
@@ -228,12 +233,22 @@ static inline uint32_t or_bound(const or_state* s, int u, int d) {
   return ub;
 }
 
-/* candidate x for vertex u is admissible (label / injectivity) */
+/* candidate x for vertex u is admissible (label / injectivity; vertex-
+ * induced: not adjacent to the image of any earlier non-neighbour, the DIFF
+ * of src/gpu_match.cu:570-587) */
 static inline int or_admit(const or_state* s, int u, uint32_t x) {
-  if (s->pl->labeled) return (1u << s->g->lab[x]) == s->pl->lbit[u];
+  const or_graph* g = s->g;
   uint32_t xm = s->pl->X[u];
-  for (int j = 0; j < u; j++)
-    if ((xm >> j & 1u) && s->phi[j] == x) return 0;
+  if (s->pl->labeled) {
+    if ((1u << g->lab[x]) != s->pl->lbit[u]) return 0;
+  } else {
+    for (int j = 0; j < u; j++)
+      if ((xm >> j & 1u) && s->phi[j] == x) return 0;
+  }
+  if (s->pl->induced)
+    for (int j = 0; j < u; j++)
+      if ((xm >> j & 1u) && or_member(g->colidx + g->rowptr[s->phi[j]], or_deg(g, s->phi[j]), x))
+        return 0;
   return 1;
 }
 
@@ -272,7 +287,7 @@ static uint64_t or_dfs(or_state* s, int d) {
   n = or_below(c, n, ub);
   uint64_t total = 0;
   if (u == pl->k - 1) {
-    if (pl->labeled) {
+    if (pl->labeled || pl->induced) {
       for (uint64_t i = 0; i < n; i++) total += or_admit(s, u, c[i]);
     } else {
       total = n;
@@ -305,6 +320,8 @@ static int or_is_job(const or_graph* g, const or_plan* pl, uint32_t r, uint32_t 
 
 /*
  * Count matches of pattern p in the CSR graph.
+ *   labeled     bit 0: labeled (LABELED=true); bit 1: vertex-induced
+ *               (EDGE_INDUCED=false).
  *   per_edge    optional u64[m]: count of the job rooted at CSR edge e (0 for
  *               edges that are not jobs), before the |Aut| multiply.
  *   edge_lo/hi  restrict the jobs to CSR edges [edge_lo, edge_hi) (hi=0: all).
@@ -317,7 +334,10 @@ int or_count(uint64_t n, uint64_t m, const uint64_t* rowptr, const uint32_t* col
              uint64_t edge_lo, uint64_t edge_hi, int threads,
              uint64_t* per_edge, uint64_t* out_count, uint64_t* out_jobs) {
   or_plan pl;
+  const int induced = (labeled >> 1) & 1;
+  labeled &= 1;
   if (or_plan_pattern(p, labeled, &pl) != 0) return -1;
+  pl.induced = induced;
   if (labeled && !labels) return -2;
   if (labeled) {
     for (uint64_t v = 0; v < n; v++) if (labels[v] > 31) return -3;

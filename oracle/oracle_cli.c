@@ -1,5 +1,5 @@
 /*
- * pm_oracle <graph-prefix> <pattern.g> [--unlabeled] [--threads N]
+ * pm_oracle <graph-prefix> <pattern.g> [--unlabeled] [--vertex-induced] [--threads N]
  *
  * TEST INFRASTRUCTURE ONLY: the CPU oracle behind the reference's CLI shape
  * (test.cu:15-212).  It reads the CSR files the way
@@ -63,12 +63,14 @@ static int read_pattern(const char* path, int labeled, or_pattern* p) {
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: pm_oracle <graph-prefix> <pattern.g> [--unlabeled] [--threads N]\n");
+    fprintf(stderr, "usage: pm_oracle <graph-prefix> <pattern.g> [--unlabeled] [--vertex-induced] "
+                    "[--threads N]\n");
     return 2;
   }
-  int labeled = 1, threads = 0;
+  int labeled = 1, induced = 0, threads = 0;
   for (int i = 3; i < argc; i++) {
     if (!strcmp(argv[i], "--unlabeled")) labeled = 0;
+    else if (!strcmp(argv[i], "--vertex-induced")) induced = 1;
     else if (!strcmp(argv[i], "--threads") && i + 1 < argc) threads = atoi(argv[++i]);
   }
   char path[4096];
@@ -91,7 +93,7 @@ int main(int argc, char** argv) {
   if (read_pattern(argv[2], labeled, &p)) { fprintf(stderr, "bad pattern %s\n", argv[2]); return 1; }
   uint64_t count = 0, jobs = 0;
   double t0 = omp_get_wtime();
-  int rc = or_count((uint64_t)n, (uint64_t)m, rowptr, colidx, labels, &p, labeled, 0, 0, threads,
+  int rc = or_count((uint64_t)n, (uint64_t)m, rowptr, colidx, labels, &p, labeled | (induced << 1), 0, 0, threads,
                     NULL, &count, &jobs);
   double t1 = omp_get_wtime();
   if (rc) { fprintf(stderr, "oracle failed (%d)\n", rc); return 1; }

@@ -24,6 +24,7 @@ CLI_PATH = os.path.join(_HERE, "bin", "test")
 
 PM_GRAPH_LABELS = 0x1
 PM_GRAPH_HOST_ONLY = 0x2
+PM_LABELED, PM_VERTEX_INDUCED = 0x1, 0x2  # pattern modes
 PM_OK, PM_ERR_IO, PM_ERR_FORMAT, PM_ERR_PATTERN = 0, -1, -2, -3
 PM_ERR_DEVICE, PM_ERR_ARG, PM_ERR_OVERFLOW, PM_ERR_NOMEM = -4, -5, -6, -7
 MAX_PATTERN = 7
@@ -117,6 +118,7 @@ class PatternInfo(C.Structure):
         ("label", C.c_int32 * MAX_PATTERN),
         ("sym_mask", C.c_uint32 * MAX_PATTERN),
         ("nbr_mask", C.c_uint32 * MAX_PATTERN),
+        ("vertex_induced", C.c_int32),
     ]
 
 
@@ -288,17 +290,22 @@ class Pattern:
     def __init__(self, handle: int):
         self._h = C.c_void_p(handle)
 
+    @staticmethod
+    def _mode(labeled: bool, induced: bool) -> int:
+        return (PM_LABELED if labeled else 0) | (PM_VERTEX_INDUCED if induced else 0)
+
     @classmethod
-    def load(cls, path: str, labeled: bool = True) -> "Pattern":
+    def load(cls, path: str, labeled: bool = True, induced: bool = False) -> "Pattern":
+        """induced=True: vertex-induced matches (EDGE_INDUCED=false)."""
         h = C.c_void_p()
-        _check(lib().pm_pattern_load(path.encode(), 1 if labeled else 0, C.byref(h)),
+        _check(lib().pm_pattern_load(path.encode(), cls._mode(labeled, induced), C.byref(h)),
                f"pattern {path}")
         return cls(h.value)
 
     @classmethod
-    def parse(cls, text: str, labeled: bool = True) -> "Pattern":
+    def parse(cls, text: str, labeled: bool = True, induced: bool = False) -> "Pattern":
         h = C.c_void_p()
-        _check(lib().pm_pattern_parse(text.encode(), 1 if labeled else 0, C.byref(h)),
+        _check(lib().pm_pattern_parse(text.encode(), cls._mode(labeled, induced), C.byref(h)),
                "pattern")
         return cls(h.value)
 

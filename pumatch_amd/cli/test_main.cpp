@@ -12,8 +12,8 @@
 
 static void usage() {
   std::fprintf(stderr,
-               "usage: test <graph-prefix> <pattern.g> [--unlabeled] [--waves N] [--chunk N]\n"
-               "            [--repeat N] [--no-result-file]\n");
+               "usage: test <graph-prefix> <pattern.g> [--unlabeled] [--vertex-induced]\n"
+               "            [--waves N] [--chunk N] [--repeat N] [--no-result-file]\n");
 }
 
 int main(int argc, char** argv) {
@@ -21,13 +21,14 @@ int main(int argc, char** argv) {
     usage();
     return 2;
   }
-  int labeled = 1, write_result = 1;
+  int labeled = 1, induced = 0, write_result = 1;
   pm_opts opts;
   pm_opts_default(&opts);
   for (int i = 3; i < argc; i++) {
     std::string a = argv[i];
     if (a == "--unlabeled") labeled = 0;
     else if (a == "--labeled") labeled = 1;
+    else if (a == "--vertex-induced") induced = 1;  // EDGE_INDUCED=false
     else if (a == "--no-result-file") write_result = 0;
     else if (a == "--waves" && i + 1 < argc) opts.waves = std::atoi(argv[++i]);
     else if (a == "--chunk" && i + 1 < argc) opts.chunk = std::atoi(argv[++i]);
@@ -48,7 +49,8 @@ int main(int argc, char** argv) {
   std::printf("graph: %llu vertices, %llu directed edges, max degree %u\n",
               (unsigned long long)gi.n, (unsigned long long)gi.m, gi.max_degree);
   pm_pattern* p = nullptr;
-  rc = pm_pattern_load(argv[2], labeled, &p);
+  rc = pm_pattern_load(argv[2], (labeled ? PM_LABELED : 0) | (induced ? PM_VERTEX_INDUCED : 0),
+                       &p);
   if (rc) {
     std::fprintf(stderr, "pattern %s: %s\n", argv[2], pm_last_error());
     return 1;

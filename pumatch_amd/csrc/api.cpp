@@ -175,6 +175,10 @@ void pm_graph_destroy(pm_graph* g) {
 
 int pm_pattern_parse(const char* text, int32_t labeled, pm_pattern** out) {
   if (!text || !out) return PM_ERR_ARG;
+  if (labeled & ~(PM_LABELED | PM_VERTEX_INDUCED)) {
+    set_error("pattern mode: unknown bits (PM_LABELED | PM_VERTEX_INDUCED)");
+    return PM_ERR_ARG;
+  }
   pm_pattern* p = new (std::nothrow) pm_pattern();
   if (!p) return PM_ERR_NOMEM;
   std::string err;
@@ -218,6 +222,7 @@ int pm_pattern_info_get(const pm_pattern* p, pm_pattern_info* info) {
     info->sym_mask[u] = pl.B[u];
     info->nbr_mask[u] = pl.A[u];
   }
+  info->vertex_induced = pl.induced;
   return PM_OK;
 }
 

@@ -113,7 +113,8 @@ struct KPlan {
   int8_t last_check;  // labeled: label test needed on the last vertex
   int8_t pair;        // fused tail over unordered adjacent pairs (see fused_tail)
   int8_t segs;        // label-grouped rows available (labeled, resident)
-  int8_t pad[3];
+  int8_t induced;     // vertex-induced (resident only; no fused tail)
+  int8_t pad[2];
   uint32_t split;       // phase-1 split threshold (list entries of work)
   uint32_t hub_factor;  // fused: owners with deg > hub_factor*|P| go one by one
   KOp ops[kMaxSlots];
@@ -532,7 +533,29 @@ __device__ __forceinline__ bool admit(const KGraph& g, const KPlan& P, const Wav
   return ok;
 }
 
+// Vertex-induced matching: x is adjacent to none of the images of the
+// earlier non-neighbours of core vertex u (the DIFF passes against their
+// rows, src/gpu_match.cu:570-587,734-738).  Resident rows only; with
+// label-grouped rows only the label-lab[u] part of each row is searched
+// (x has passed the label test).  Lane-local.
+__device__ __forceinline__ bool outside(const KGraph& g, const KPlan& P, const WaveState& ws,
+                                        int u, uint32_t x) {
+  uint32_t xm = P.X[u];
+  const int seg = (P.labeled && P.segs) ? (int)P.lab[u] : -1;
+  bool ok = true;
+  while (xm && ok) {
+    const int j = __builtin_ctz(xm);
+    xm &= xm - 1;
+    const uint32_t* a;
+    uint32_t na;
+    rows<false>(g, ws.phi[j], seg, &a, &na);
+    ok = !member(a, na, x);
+  }
+  return ok;
+}
+
 // next admitted candidate of core vertex d+1 and its index; false when done
+template <bool PG>
 __device__ __forceinline__ bool next_cand(const KGraph& g, const KPlan& P, WaveState& ws, int d,
                                           uint32_t lane, uint32_t& x, uint32_t& ix, Counters& ct) {
   for (;;) {
@@ -556,6 +579,7 @@ __device__ __forceinline__ bool next_cand(const KGraph& g, const KPlan& P, WaveS
     if (ok) {
       v = c[idx];
       ok = admit(g, P, ws, d + 1, P.cand_check[d], v);
+      if (!PG && P.induced && ok) ok = outside(g, P, ws, d + 1, v);
     }
     ws.cval[d][lane] = v;
     ct.alg += 4ull * min(n - base, (uint32_t)kWave);
@@ -629,6 +653,61 @@ __device__ unsigned long long last_count(const KGraph& g, const KPlan& P, WaveSt
   }
   cnt -= (uint32_t)__popcll(__ballot(in));
   return cnt;
+}
+
+// Vertex-induced count of the last core vertex once 0..k-2 are fixed
+// (resident rows).  Each element of its list — or of the shorter of the two
+// lists it is the intersection of — is tested lane by lane: membership in
+// the other list, label, injectivity (unlabeled) and non-adjacency to the
+// images of its earlier non-neighbours.
+__device__ unsigned long long last_count_induced(const KGraph& g, const KPlan& P, WaveState& ws,
+                                                 uint32_t lane, Counters& ct) {
+  const int u = P.k - 1;
+  const int dl = P.k - 2;
+  const uint32_t ub = P.labeled ? kInf : vertex_bound(P, ws, u, dl);
+  const bool segs = P.labeled && P.segs;
+  const uint32_t* a;
+  uint32_t na;
+  if (P.last_kind == 0) {
+    a = uniptr(ws.sptr[P.last_src]);
+    na = uni(ws.slen[P.last_src]);
+  } else {
+    rows<false>(g, uni(ws.phi[dl]), segs ? (int)P.lab[u] : -1, &a, &na);
+    a = uniptr(a);
+    na = uni(na);
+  }
+  if (ub != kInf) na = uni(below(a, na, ub));
+  const uint32_t* b = nullptr;
+  uint32_t nb = 0;
+  if (P.last_kind == 2) {
+    b = uniptr(ws.sptr[P.last_src]);
+    nb = uni(ws.slen[P.last_src]);
+    if (ub != kInf) nb = uni(below(b, nb, ub));
+    if (nb < na) {
+      const uint32_t* t = a;
+      a = b;
+      b = t;
+      const uint32_t tn = na;
+      na = nb;
+      nb = tn;
+    }
+  }
+  // label-exact lists need no label test (as in last_count)
+  const bool check = P.labeled && (P.last_kind == 0 ? P.last_check != 0 : !segs);
+  unsigned long long s = 0;
+  for (uint32_t base = 0; base < na; base += kWave) {
+    const uint32_t i = base + lane;
+    if (i < na) {
+      const uint32_t y = a[i];
+      bool ok = !b || member(b, nb, y);
+      ok = ok && admit(g, P, ws, u, check, y);
+      ok = ok && outside(g, P, ws, u, y);
+      s += ok ? 1ull : 0ull;
+    }
+  }
+  ct.alg += 4ull * ((uint64_t)na + nb);
+  ct.nops++;
+  return wave_sum(s);
 }
 
 // Write one deferred piece (uniform call): depth and the candidate range
@@ -1016,9 +1095,13 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
     emit_one(O, ws, 0, 0, 0, uni(ws.blk), lane);
     return 0;
   }
+  // vertex-induced plans (resident) enumerate up to vertex k-2 and count the
+  // last vertex with last_count_induced; they never take the fused tail
+  const bool induced = !PG && P.induced;
   if (k == 3) {
-    bool ok;
-    const unsigned long long v = last_count<PG>(g, P, ws, lane, ct, &ok);
+    bool ok = true;
+    const unsigned long long v = induced ? last_count_induced(g, P, ws, lane, ct)
+                                         : last_count<PG>(g, P, ws, lane, ct, &ok);
     if (!ok) {
       emit_one(O, ws, 0, 0, 0, uni(ws.blk), lane);
       return 0;
@@ -1109,7 +1192,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
   ws.cmask[1] = 0;
   int d = 1, stop = 1;  // stop: depth whose exhaustion ends this search
   unsigned long long cnt = 0;
-  bool fused_now = (k == 4);  // the fused level is depth k-3
+  bool fused_now = (k == 4) && !induced;  // the fused level is depth k-3
   if (resume && uni(ws.rdepth) >= 1) {
     // rebuild the path phi(2..D) from the recorded candidate indices
     const int D = (int)uni(ws.rdepth);
@@ -1145,7 +1228,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
     ws.cmask[D] = 0;
     d = D;
     stop = D;
-    fused_now = (d == k - 3);
+    fused_now = (d == k - 3) && !induced;
   }
   // the rest of the search above depth d: siblings after the current
   // candidate at every depth stop..d-1, one piece each
@@ -1169,7 +1252,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
       continue;
     }
     uint32_t x, ix;
-    if (!next_cand(g, P, ws, d, lane, x, ix, ct)) {
+    if (!next_cand<PG>(g, P, ws, d, lane, x, ix, ct)) {
       if (d == stop) break;
       d--;
       continue;
@@ -1177,6 +1260,10 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
     ws.cidx[d] = ix;
     ws.phi[d + 1] = x;
     ws.wacc[d + 1] = uni64(ws.wacc[d]) * wt_of(P, d + 1, x);
+    if (induced && d + 1 == k - 2) {
+      cnt += uni64(ws.wacc[d + 1]) * last_count_induced(g, P, ws, lane, ct);
+      continue;
+    }
     if (!run_ops<PG>(g, P, ws, d + 1, arena, cap, lane, ct)) {
       if (uni64(*(volatile unsigned long long*)O.count) > O.soft) {
         // too many pieces already: stop, leaving the rest as a few pieces
@@ -1190,7 +1277,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
     }
     setup_cand(P, ws, d + 1);
     d++;
-    fused_now = (d == k - 3);
+    fused_now = (d == k - 3) && !induced;
   }
   return cnt;
 }
@@ -1785,6 +1872,7 @@ void build_kplan(const Plan& p, bool segs, KPlan* kp, int* nregions) {
   kp->labeled = p.labeled;
   kp->rc_order = p.rc_order;
   kp->segs = (p.labeled && segs) ? 1 : 0;
+  kp->induced = p.induced ? 1 : 0;
   kp->deg0 = p.deg[0];
   kp->deg1 = p.deg[1];
   kp->deg2 = p.deg[2];
@@ -1850,7 +1938,7 @@ void build_kplan(const Plan& p, bool segs, KPlan* kp, int* nregions) {
     // label.  Then the ordered pairs (x,y) are the adjacent pairs inside P,
     // and each unordered pair can be visited once (y > x) and credited in
     // both roles.
-    if (p.labeled && kp->segs && kc >= 4 && kp->last_kind == 2 &&
+    if (p.labeled && !p.induced && kp->segs && kc >= 4 && kp->last_kind == 2 &&
         kp->last_src == kp->cand_slot[kc - 3] && p.clabel[kc - 2] == p.clabel[kc - 1] &&
         kp->cand_check[kc - 3] == 0)
       kp->pair = 1;
@@ -2427,6 +2515,12 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
                        pm_result* res) {
   if (npat <= 0) return PM_OK;
   const bool paged = opts->cache_bytes > 0;
+  if (paged)
+    for (int q = 0; q < npat; q++)
+      if (plans[q]->induced) {
+        set_error("vertex-induced matching needs the rows resident (cache_bytes = 0)");
+        return PM_ERR_ARG;
+      }
   int rc = device_upload_rows(g, !paged);
   if (rc) return rc;
   // learned parts: model predictions per vertex (cached per graph and model)

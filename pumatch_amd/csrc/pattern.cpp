@@ -50,11 +50,11 @@ int parse_pattern_text(const std::string& text, int labeled, Plan* out, std::str
       *err = "pattern has more than 7 vertices (PAT_SIZE)";
       return PM_ERR_PATTERN;
     }
-    if (labeled && (lab < 0 || lab > 31)) {
+    if ((labeled & PM_LABELED) && (lab < 0 || lab > 31)) {
       *err = "vertex label outside 0..31";
       return PM_ERR_PATTERN;
     }
-    p.vlabel[k] = labeled ? (int)lab : 1;
+    p.vlabel[k] = (labeled & PM_LABELED) ? (int)lab : 1;
     k++;
     i++;
   }
@@ -80,7 +80,8 @@ int parse_pattern_text(const std::string& text, int labeled, Plan* out, std::str
     p.adj[b][a] = (int)lab;
   }
   p.k = k;
-  p.labeled = labeled ? 1 : 0;
+  p.labeled = (labeled & PM_LABELED) ? 1 : 0;
+  p.induced = (labeled & PM_VERTEX_INDUCED) ? 1 : 0;
   *out = p;
   return PM_OK;
 }
@@ -163,12 +164,14 @@ int build_plan(Plan* pp, std::string* err) {
   }
   p.rc_order = (!p.labeled && (p.B[1] & 1u)) ? 1 : 0;
 
-  // ---- folding of pendant trees (homomorphism semantics only) ----
+  // ---- folding of pendant trees (homomorphism semantics only; a
+  // vertex-induced pendant vertex must also avoid the other vertices'
+  // neighbourhoods, so nothing is folded there) ----
   bool alive[kMaxK];
   for (int u = 0; u < k; u++) alive[u] = true;
   p.nfold = 0;
   for (int u = 0; u < k; u++) p.fold_children[u] = 0;
-  if (p.labeled) {
+  if (p.labeled && !p.induced) {
     bool changed = true;
     while (changed) {
       changed = false;

@@ -98,6 +98,7 @@ struct Plan {
   // ---- the pattern as read (file order = matching order) ----
   int k = 0;
   int labeled = 1;
+  int induced = 0;               // vertex-induced (EDGE_INDUCED=false)
   long long aut = 1;             // PatternMultiplicity, src/pattern.h:236
   int rc_order = 0;              // unlabeled job filter r > c
   int deg[kMaxK] = {};           // pattern degree (job filter uses deg[0..2])

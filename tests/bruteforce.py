@@ -10,6 +10,8 @@ unlabeled: the number of injective maps carrying every pattern edge onto a
            graph edge (embeddings), which the reference's
            |Aut| x symmetry-broken count equals when vertex 2 is adjacent to
            vertex 1 (test.cu:198).
+induced:   (either mode) every pattern non-edge must also land on a pair of
+           non-adjacent graph vertices (EDGE_INDUCED=false).
 """
 from itertools import permutations, product
 
@@ -23,12 +25,17 @@ def pattern_edges(adj, k):
     return [(i, j) for i in range(k) for j in range(i) if adj[i][j]]
 
 
-def count_labeled(rowptr, colidx, labels, k, padj, plab):
+def pattern_non_edges(adj, k):
+    return [(i, j) for i in range(k) for j in range(i) if not adj[i][j]]
+
+
+def count_labeled(rowptr, colidx, labels, k, padj, plab, induced=False):
     nb = adjacency(rowptr, colidx)
     n = len(nb)
     deg = [len(s) for s in nb]
     pdeg = [sum(1 for j in range(k) if padj[i][j] > 0) for i in range(k)]
     edges = pattern_edges(padj, k)
+    non = pattern_non_edges(padj, k) if induced else []
     total = 0
     for r in range(n):
         for c in sorted(nb[r]):
@@ -43,18 +50,21 @@ def count_labeled(rowptr, colidx, labels, k, padj, plab):
                 phi = (r, c) + rest
                 if any(labels[phi[u]] != plab[u] for u in range(2, k)):
                     continue
-                if all(phi[j] in nb[phi[i]] for i, j in edges):
+                if all(phi[j] in nb[phi[i]] for i, j in edges) and \
+                        not any(phi[j] in nb[phi[i]] for i, j in non):
                     total += 1
     return total
 
 
-def count_embeddings(rowptr, colidx, k, padj):
+def count_embeddings(rowptr, colidx, k, padj, induced=False):
     nb = adjacency(rowptr, colidx)
     n = len(nb)
     edges = pattern_edges(padj, k)
+    non = pattern_non_edges(padj, k) if induced else []
     total = 0
     for phi in permutations(range(n), k):
-        if all(phi[j] in nb[phi[i]] for i, j in edges):
+        if all(phi[j] in nb[phi[i]] for i, j in edges) and \
+                not any(phi[j] in nb[phi[i]] for i, j in non):
             total += 1
     return total
 

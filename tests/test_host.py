@@ -120,6 +120,20 @@ def test_pattern_errors():
         pm.Pattern.parse("v 0 40\nv 1 1\nv 2 1\ne 0 1 1\ne 1 2 1\n")
 
 
+def test_vertex_induced_mode_flag():
+    text = g_text(7, edges_to_adj(7, C1_EDGES))
+    p = pm.Pattern.parse(text, True, induced=True)
+    assert p.info().vertex_induced == 1 and p.info().labeled == 1
+    q = pm.Pattern.parse(text, True)
+    assert q.info().vertex_induced == 0
+    # vertex-induced plans fold nothing: the pendant path stays in the core
+    # and needs slots of its own
+    assert p.info().nslots > q.info().nslots
+    assert pm.Pattern.parse(text, False, induced=True).info().vertex_induced == 1
+    h = pm.C.c_void_p()
+    assert pm.lib().pm_pattern_parse(text.encode(), 4, pm.C.byref(h)) == pm.PM_ERR_ARG
+
+
 def test_folding_of_pendant_trees():
     # labeled K4 + path 3-4-5-6: the path folds onto vertex 3, the core is K4
     p = pm.Pattern.parse(g_text(7, edges_to_adj(7, C1_EDGES)), True)
