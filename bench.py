@@ -27,8 +27,10 @@ Two more objects compare plain and learned+batched schedules:
   "learned": the MLP trained on traces of seeds 7-9 and evaluated on seed
     10: learned job order on permuted RMAT scale 15 (resident), and learned
     prefetch + eviction, alone and in one batched launch, out of core on
-    permuted Chung-Lu graphs (50k vertices) with 40% of the rows in HBM,
-    each against the plain schedule.
+    permuted Chung-Lu graphs (200k vertices, max degree 5000) with 40% of
+    the rows in HBM, each against the plain schedule (LRU, rows of <= 1024
+    entries read from host memory when not cached); the deferral-only
+    schedule (no direct reads) is timed beside it.
 """
 import argparse
 import json
@@ -193,11 +195,13 @@ def learned_section(pm):
                       "speedup": round(tp / tl, 3), "per_pattern_plain_learned_ms": per}
     out["order"] = order
 
-    # (2) learned package ranking out of core, (3) batched
-    N, PKG = 50000, 16384
+    # (2) learned package ranking out of core, (3) batched.  Max degree
+    # 5000 as in the 65.6M-vertex configuration, so the hub rows (> 1024
+    # entries, the direct-read limit) page through the cache
+    N, DMAX, MEAN, PKG = 200000, 5000, 30, 16384
 
     def chunglu(seed):
-        g0 = pm.Graph.chung_lu(N, 2.2, N // 100, 20, seed=seed, device=False)
+        g0 = pm.Graph.chung_lu(N, 2.2, DMAX, MEAN, seed=seed, device=False)
         g = permuted(g0, seed + 100)
         g.set_uniform_labels(8, seed)
         return g
@@ -218,19 +222,24 @@ def learned_section(pm):
     mp.train(np.concatenate(Xs), np.concatenate(Ys).astype(np.float32), epochs=30, lr=3e-3)
     g = chunglu(10)
     cache = int(g.m * 4 * 0.4) // PKG * PKG
-    paging = {"workload": (f"chunglu50k-perm: Chung-Lu n={N}, exponent 2.2, max degree {N // 100}, "
-                           f"mean 20, ids permuted; {cache} B HBM cache of {PKG} B packages "
-                           "(40% of rows); unlabeled triangle and K4"), "eval_seed": 10,
+    paging = {"workload": (f"chunglu200k-perm: Chung-Lu n={N}, exponent 2.2, max degree {DMAX}, "
+                           f"mean {MEAN}, ids permuted; {cache} B HBM cache of {PKG} B packages "
+                           "(40% of rows); rows of <= 1024 entries read from pinned host "
+                           "memory when not cached; unlabeled triangle and K4"), "eval_seed": 10,
               "patterns": {}}
     tp = tl = 0.0
     for name, p in ppats:
+        res = pm.match(g, p, repeats=3)
         a = pm.match(g, p, cache_bytes=cache, package_bytes=PKG)
         b = pm.match(g, p, cache_bytes=cache, package_bytes=PKG, policy="learned", prefetch=True,
                      model=mp)
-        assert a.count == b.count
+        c = pm.match(g, p, cache_bytes=cache, package_bytes=PKG, direct_degree=-1)
+        assert a.count == b.count == c.count == res.count
         paging["patterns"][name] = {
+            "resident_ms": round(res.min_ms, 2),
             "plain_ms": round(a.kernel_ms, 2), "learned_ms": round(b.kernel_ms, 2),
-            "plain_rounds": a.rounds, "learned_rounds": b.rounds,
+            "no_direct_reads_ms": round(c.kernel_ms, 2),
+            "plain_rounds": a.rounds, "learned_rounds": b.rounds, "no_direct_rounds": c.rounds,
             "plain_link_MB": round(a.link_bytes / 1e6, 2), "learned_link_MB": round(b.link_bytes / 1e6, 2),
             "link_GBps_during_copies": round(a.link_bytes / max(a.link_ms, 1e-9) / 1e6, 2),
             "count": a.count}

@@ -80,6 +80,13 @@ typedef struct {
                            bytes) of the jobs rooted at each vertex */
   uint32_t* package_use; /* trace, optional host u32[n]: out-of-core rounds in
                            which each vertex's package was used (0 resident) */
+  int32_t direct_degree; /* out-of-core: a missing row of at most this many
+                           entries is read straight from pinned host memory
+                           instead of deferring its search, as the reference
+                           reads rows of <= DEGREE_THRESHOLD entries through
+                           managed memory (src/config.h:41,
+                           src/gpu_match.cu:478,601,663,752).  0 = 1024 (the
+                           reference's value), < 0 = off */
 } pm_opts;
 
 #define PM_LEARN_ORDER 0x1  /* phase-1 jobs heaviest predicted root first */

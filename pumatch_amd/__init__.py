@@ -55,6 +55,7 @@ class Opts(C.Structure):
         ("model", C.c_void_p),
         ("root_work", C.POINTER(C.c_uint64)),
         ("package_use", C.POINTER(C.c_uint32)),
+        ("direct_degree", C.c_int32),
     ]
 
 
@@ -429,7 +430,7 @@ POLICY = {"lru": 0, "fifo": 1, "learned": 2}
 
 def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
                stream, cache_bytes, package_bytes, policy, prefetch, order="cost", sample=0,
-               model=None, learned=0):
+               model=None, learned=0, direct_degree=0):
     o = Opts()
     lib().pm_opts_default(C.byref(o))
     o.waves, o.chunk, o.repeats = waves, chunk, repeats
@@ -439,6 +440,7 @@ def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge
     o.edge_lo, o.edge_hi = edge_lo, edge_hi
     o.order = ORDER[order]
     o.sample = sample
+    o.direct_degree = direct_degree
     if model is not None:
         o.model = model._h.value
         o.learned = learned if learned else (PM_LEARN_ORDER | PM_LEARN_PAGING)
@@ -464,16 +466,18 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
           per_edge: bool = False, stream: int | None = None, cache_bytes: int = 0,
           package_bytes: int = 0, policy: str = "lru", prefetch: bool = False,
           order: str = "cost", sample: int = 0, model=None, learned: int = 0,
-          traces: bool = False) -> MatchResult:
+          traces: bool = False, direct_degree: int = 0) -> MatchResult:
     """Count matches of p in g on the GPU (pm_match).  order="file" matches
     labeled patterns in file order as the reference does; "cost" (default)
     lets the planner pick the order of vertices 2.. (same count).  With a
     Model, `learned` (PM_LEARN_* bits, default both) turns on learned job
     order and package ranking.  traces=True records root_work and
-    package_use (both per vertex) into the result."""
+    package_use (both per vertex) into the result.  direct_degree (out of
+    core): missing rows of at most that many entries are read from pinned
+    host memory instead of being deferred (0: 1024, < 0: never)."""
     o, pe = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
                        stream, cache_bytes, package_bytes, policy, prefetch, order, sample,
-                       model, learned)
+                       model, learned, direct_degree)
     rw = pu = None
     if traces:
         rw = np.zeros(max(g.n, 1), dtype=np.uint64)
@@ -493,7 +497,7 @@ def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: 
                 split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
                 stream: int | None = None, cache_bytes: int = 0, package_bytes: int = 0,
                 policy: str = "lru", prefetch: bool = False, order: str = "cost",
-                sample: int = 0, model=None, learned: int = 0) -> list:
+                sample: int = 0, model=None, learned: int = 0, direct_degree: int = 0) -> list:
     """Count several patterns in one persistent launch (pm_match_batch).
     Returns one MatchResult per pattern; count/jobs are the pattern's own,
     the timing fields describe the whole launch."""
@@ -501,7 +505,7 @@ def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: 
     n = len(patterns)
     o, _ = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, False,
                       stream, cache_bytes, package_bytes, policy, prefetch, order, sample,
-                      model, learned)
+                      model, learned, direct_degree)
     hs = (C.c_void_p * max(n, 1))(*[q._h.value for q in patterns])
     rs = (Result * max(n, 1))()
     _check(lib().pm_match_batch(g._h, hs, n, C.byref(o), rs), "match_batch")

@@ -143,6 +143,7 @@ struct KGraph {
   unsigned long long* miss_count;
   uint32_t miss_cap;
   uint8_t* used;  // per cache slot: read during this round
+  uint32_t direct_max;  // a missing row of <= this many entries is read from jcol
 };
 
 struct Task {
@@ -292,6 +293,14 @@ __device__ __forceinline__ bool rows(const KGraph& g, uint32_t v, int seg, const
   const uint32_t pk = g.pkg_of[v];
   const int32_t s = g.pkg_slot[pk];
   if (s < 0) {
+    // short rows come straight from pinned host memory (the rows of the
+    // edge scan) instead of deferring the search
+    const uint32_t n = (uint32_t)(g.rowptr[v + 1] - r0);
+    if (n <= g.direct_max) {
+      *p = g.jcol + r0;
+      *len = n;
+      return true;
+    }
     if (missing) *missing = pk;
     const unsigned int bit = 1u << (pk & 31);
     if (!(atomicOr(&g.miss_bits[pk >> 5], bit) & bit)) {
@@ -1709,23 +1718,34 @@ __global__ __launch_bounds__(256) void k_copy_packages(const uint32_t* __restric
 }
 
 // Pending pieces stay in HBM between rounds.  Needs of a piece: the
-// packages of its path rows phi[0..depth] and the package that stopped it.
+// packages of its path rows phi[0..depth] (rows read directly from host
+// memory, of <= direct_max entries, need none) and the package that
+// stopped it.
 constexpr int kNeed = kMaxDepth + 2;  // count + packages
 
-__device__ __forceinline__ int piece_needs(const Resume& d, const uint32_t* pkg_of, uint32_t* q) {
+struct NeedArgs {
+  const uint32_t* pkg_of;
+  const uint64_t* rowptr;
+  uint32_t direct_max;
+};
+
+__device__ __forceinline__ int piece_needs(const Resume& d, const NeedArgs& A, uint32_t* q) {
   int c = 0;
-  for (int j = 0; j <= d.depth && j < kMaxDepth; j++) q[c++] = pkg_of[d.phi[j]];
+  for (int j = 0; j <= d.depth && j < kMaxDepth; j++) {
+    const uint32_t v = d.phi[j];
+    if (A.rowptr[v + 1] - A.rowptr[v] > A.direct_max) q[c++] = A.pkg_of[v];
+  }
   q[c++] = d.blk;
   return c;
 }
 
 // needs of pieces [0, n) for the host's admission (row i: count, packages)
-__global__ void k_piece_needs(const Resume* __restrict__ P, uint64_t n,
-                              const uint32_t* __restrict__ pkg_of, uint32_t* __restrict__ out) {
+__global__ void k_piece_needs(const Resume* __restrict__ P, uint64_t n, NeedArgs A,
+                              uint32_t* __restrict__ out) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t q[kNeed];
-    const int c = piece_needs(P[i], pkg_of, q);
+    const int c = piece_needs(P[i], A, q);
     out[i * kNeed] = (uint32_t)c;
     for (int j = 0; j < c; j++) out[i * kNeed + 1 + j] = q[j];
   }
@@ -1736,7 +1756,7 @@ __global__ void k_piece_needs(const Resume* __restrict__ P, uint64_t n,
 // cnt[1] (keep); one atomic per wave for each.  Pieces keep their relative
 // order within a wave only; the rounds' progress does not depend on it.
 __global__ __launch_bounds__(256) void k_piece_select(const Resume* __restrict__ P, uint64_t n,
-                                                      const uint32_t* __restrict__ pkg_of,
+                                                      NeedArgs A,
                                                       const int32_t* __restrict__ pkg_slot,
                                                       Resume* __restrict__ run,
                                                       Resume* __restrict__ keep,
@@ -1751,7 +1771,7 @@ __global__ __launch_bounds__(256) void k_piece_select(const Resume* __restrict__
     if (live) {
       d = P[i];
       uint32_t q[kNeed];
-      const int c = piece_needs(d, pkg_of, q);
+      const int c = piece_needs(d, A, q);
       here = true;
       for (int j = 0; j < c; j++) here = here && pkg_slot[q[j]] >= 0;
     }
@@ -2314,6 +2334,7 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
   rc = fetch_status();
   if (rc) return rc;
   bool selected = false;  // the last round ran the selection (its keep count is pending)
+  const NeedArgs need_args = {L.g.pkg_of, L.g.rowptr, L.g.direct_max};
   for (;;) {
     unsigned long long cur8[8];
     std::memcpy(cur8, g_ws.h_stat, sizeof(cur8));
@@ -2322,10 +2343,6 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
     const uint64_t nd = cur8[4];
     if (nd > rcap) {
       set_error("out-of-core: more deferred pieces in one round than the buffer holds");
-      return PM_ERR_OVERFLOW;
-    }
-    if (cur8[2] > g_ws.task_cap) {
-      set_error("out-of-core: range task buffer overflow");
       return PM_ERR_OVERFLOW;
     }
     rs.misses += mc;
@@ -2349,8 +2366,11 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
       npend += nd;
     }
     const bool edges_left = cur8[0] < span;
-    const uint64_t tasks_done = std::min(cur8[1], cur8[2]);
-    const bool tasks_left = L.tasks && tasks_done < cur8[2];
+    // tasks past the buffer's end were never written (their jobs kept the
+    // work, see run_job), so only the first task_cap exist
+    const uint64_t tasks_made = std::min<uint64_t>(cur8[2], g_ws.task_cap);
+    const uint64_t tasks_done = std::min<uint64_t>(cur8[1], tasks_made);
+    const bool tasks_left = L.tasks && tasks_done < tasks_made;
     const uint8_t* used = g_ws.h_used;
     if (!use_acc.empty()) {
       // trace: each package used this round counts once
@@ -2369,7 +2389,7 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
     const uint64_t nw = std::min(npend, kWindow);
     if (nw) {
       hipLaunchKernelGGL(k_piece_needs, dim3((unsigned)std::min<uint64_t>((nw + 255) / 256, 1024)),
-                         dim3(256), 0, st, (const Resume*)g_ws.pend[pc], nw, pg->d_pkg_of,
+                         dim3(256), 0, st, (const Resume*)g_ws.pend[pc], nw, need_args,
                          (uint32_t*)g_ws.d_needs);
       HIP_OK(hipMemcpyAsync(g_ws.h_needs, g_ws.d_needs, sizeof(uint32_t) * nw * kNeed,
                             hipMemcpyDeviceToHost, st));
@@ -2451,7 +2471,7 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
       hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor + 6, 2, 0ull);
       hipLaunchKernelGGL(k_piece_select,
                          dim3((unsigned)std::min<uint64_t>((npend + 255) / 256, 16384)),
-                         dim3(256), 0, st, (const Resume*)g_ws.pend[pc], npend, pg->d_pkg_of,
+                         dim3(256), 0, st, (const Resume*)g_ws.pend[pc], npend, need_args,
                          pg->d_pkg_slot, (Resume*)g_ws.resume[0], (Resume*)g_ws.pend[1 - pc],
                          g_ws.cursor + 6);
       pc = 1 - pc;
@@ -2705,6 +2725,9 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     kg.miss_count = pg->d_miss_count;
     kg.miss_cap = pg->miss_cap;
     kg.used = pg->d_used;
+    kg.direct_max = opts->direct_degree > 0    ? (uint32_t)opts->direct_degree
+                    : opts->direct_degree == 0 ? 1024u
+                                               : 0u;
   }
   Launch L;
   std::memset(&L, 0, sizeof(L));

@@ -79,10 +79,13 @@ def test_batch_paged():
     pats = [pm.Pattern.parse(t, lab) for t, lab in texts]
     base = [r.count for r in pm.match_batch(g, pats)]
     print(f"[batch paged] {len(pats)} patterns", flush=True)
-    rs = pm.match_batch(g, pats, cache_bytes=120 * 4096, package_bytes=4096)
+    rs = pm.match_batch(g, pats, cache_bytes=120 * 4096, package_bytes=4096, direct_degree=-1)
     assert [r.count for r in rs] == base
     assert rs[0].rounds > 1 and rs[0].link_bytes > 0
     singles = [pm.match(g, p, cache_bytes=120 * 4096, package_bytes=4096).count for p in pats]
+    # and with short rows read directly (the default)
+    assert [r.count for r in pm.match_batch(g, pats, cache_bytes=120 * 4096,
+                                            package_bytes=4096)] == singles
     assert singles == base
 
 

@@ -84,7 +84,7 @@ def test_learned_order_and_paging_keep_counts():
         assert r.count == base.count and np.array_equal(r.per_edge, base.per_edge)
         for prefetch in (False, True):
             r = pm.match(g, p, cache_bytes=120 * 4096, package_bytes=4096, policy="learned",
-                         prefetch=prefetch, model=m)
+                         prefetch=prefetch, model=m, direct_degree=-1)
             assert r.count == base.count and r.rounds > 1
 
 
@@ -99,7 +99,7 @@ def test_traces():
     assert r.root_work.sum() > 0 and r.root_work.sum() <= r.alg_bytes
     assert np.all(r.root_work[roots] > 0)
     assert not np.any(r.package_use)  # resident: no packages
-    q = pm.match(g, p, traces=True, cache_bytes=120 * 4096, package_bytes=4096)
+    q = pm.match(g, p, traces=True, cache_bytes=120 * 4096, package_bytes=4096, direct_degree=-1)
     assert q.count == r.count
     assert q.package_use.max() >= 1 and q.package_use.max() <= q.rounds
 

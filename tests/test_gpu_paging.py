@@ -38,7 +38,9 @@ def test_paged_equals_resident(graph, name, edges, labeled, policy):
     p = pm.Pattern.parse(g_text(k, edges_to_adj(k, edges)), labeled)
     base = pm.match(graph, p, per_edge=True)
     print(f"[paging] {name} labeled={labeled} policy={policy}", flush=True)
-    r = pm.match(graph, p, per_edge=True, cache_bytes=SLOTS * PKG, package_bytes=PKG, policy=policy)
+    # direct reads off: every missing row defers its search
+    r = pm.match(graph, p, per_edge=True, cache_bytes=SLOTS * PKG, package_bytes=PKG, policy=policy,
+                 direct_degree=-1)
     print(f"[paging]   rounds={r.rounds} misses={r.misses} deferred={r.deferred} "
           f"link={r.link_bytes} ms={r.kernel_ms:.1f}", flush=True)
     assert r.count == base.count
@@ -51,7 +53,8 @@ def test_paging_forces_rounds_and_matches_oracle(graph):
     for edges, labeled in ((HT_EDGES, True), ([(0, 1), (1, 2), (0, 2)], False)):
         text = g_text(7 if len(edges) > 3 else 3, edges_to_adj(7 if len(edges) > 3 else 3, edges))
         p = pm.Pattern.parse(text, labeled)
-        r = pm.match(graph, p, per_edge=True, cache_bytes=SLOTS * PKG, package_bytes=PKG)
+        r = pm.match(graph, p, per_edge=True, cache_bytes=SLOTS * PKG, package_bytes=PKG,
+                     direct_degree=-1)
         want, jobs, pe = oracle.count(rp, ci, lb, oracle.parse_g(text, labeled), labeled,
                                       per_edge=True)
         assert r.count == want and r.jobs == jobs and np.array_equal(r.per_edge, pe)
@@ -80,3 +83,22 @@ def test_cache_too_small_is_an_error(graph):
     p = pm.Pattern.parse(g_text(3, edges_to_adj(3, [(0, 1), (1, 2)])), True)
     with pytest.raises(pm.PumatchError):
         pm.match(graph, p, cache_bytes=8 * 4096, package_bytes=4096)
+
+
+@pytest.mark.parametrize("direct", [16, 64, 1024])
+@pytest.mark.parametrize("name,edges,labeled", CASES)
+def test_direct_short_rows_equal_resident(graph, name, edges, labeled, direct):
+    """Missing rows of <= direct entries are read from pinned host memory
+    (no deferral for them); counts and per-edge counts stay exact."""
+    k = max(max(e) for e in edges) + 1
+    p = pm.Pattern.parse(g_text(k, edges_to_adj(k, edges)), labeled)
+    base = pm.match(graph, p, per_edge=True)
+    print(f"[paging direct] {name} labeled={labeled} direct={direct}", flush=True)
+    r = pm.match(graph, p, per_edge=True, cache_bytes=SLOTS * PKG, package_bytes=PKG,
+                 direct_degree=direct)
+    print(f"[paging direct]   rounds={r.rounds} misses={r.misses} deferred={r.deferred}",
+          flush=True)
+    assert r.count == base.count
+    assert np.array_equal(r.per_edge, base.per_edge)
+    if direct >= 168:  # every row of the graph is short enough: nothing misses
+        assert r.misses == 0 and r.deferred == 0

@@ -159,7 +159,7 @@ def test_cli_matches_oracle_cli(tmp_path, patterns_dir):
     prefix = str(tmp_path / "rmat9")
     g.write(prefix)
     pat = os.path.join(patterns_dir, "7.g")
-    for extra in ([], ["--unlabeled"]):
+    for extra in ([], ["--unlabeled"], ["--vertex-induced"], ["--unlabeled", "--vertex-induced"]):
         a = subprocess.run([pm.CLI_PATH, prefix, pat, "--no-result-file"] + extra,
                            capture_output=True, text=True, timeout=120)
         b = subprocess.run([oracle.CLI_PATH, prefix, pat] + extra, capture_output=True, text=True,

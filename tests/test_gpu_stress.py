@@ -65,6 +65,7 @@ def test_random_out_of_core_and_learned(seed):
     for kw in (dict(policy="lru"), dict(policy="fifo"),
                dict(policy="learned", prefetch=True, model=m),
                dict(model=m, learned=pm.PM_LEARN_ORDER)):
+        kw["direct_degree"] = int(rng.choice([-1, -1, 0, 8, 40]))
         print(f"[stress ooc] seed={seed} slots={slots} {kw}", flush=True)
         got = [r.count for r in pm.match_batch(g, pats, cache_bytes=slots * 4096,
                                                 package_bytes=4096, **kw)]
