@@ -26,11 +26,15 @@ def test_random_options_match_oracle(seed):
     g = permuted(g0, seed)
     g.set_uniform_labels(int(rng.integers(2, 5)), seed)
     rp, ci, lb = g.csr()
-    texts = [(t, True) for t in batch_patterns(3, seed=200 + seed, nlabels=4, kmax=6)]
+    texts = [(t, True, False) for t in batch_patterns(3, seed=200 + seed, nlabels=4, kmax=6)]
     k = int(rng.integers(3, 6))
-    texts.append((g_text(k, random_pattern(k, 0.5, seed)), False))
-    pats = [pm.Pattern.parse(t, lab) for t, lab in texts]
-    want = [oracle.count(rp, ci, lb, oracle.parse_g(t, lab), lab)[0] for t, lab in texts]
+    texts.append((g_text(k, random_pattern(k, 0.5, seed)), False, False))
+    # vertex-induced variants of one labeled and one unlabeled pattern
+    texts.append((texts[0][0], True, True))
+    texts.append((texts[-2][0], False, True))
+    pats = [pm.Pattern.parse(t, lab, induced=ind) for t, lab, ind in texts]
+    want = [oracle.count(rp, ci, lb, oracle.parse_g(t, lab), lab, induced=ind)[0]
+            for t, lab, ind in texts]
     for trial in range(3):
         kw = {}
         if rng.random() < 0.5:
