@@ -1804,12 +1804,18 @@ struct Workspace {
   hipStream_t copy_stream = nullptr;  // host->HBM package copies
   void* arena = nullptr;
   size_t arena_bytes = 0;
-  unsigned long long* cursor = nullptr;  // 8 words
-  unsigned long long* out = nullptr;     // 8 words
+  // One block for everything a launch reports, so that it is cleared by
+  // one fill and read back by one copy: cursor (8 words), out (8 words),
+  // pcount (count, jobs per pattern).
+  void* resblk = nullptr;
+  size_t resblk_bytes = 0;
+  unsigned long long* cursor = nullptr;  // views into resblk
+  unsigned long long* out = nullptr;
+  unsigned long long* pcount = nullptr;
+  unsigned long long* h_res = nullptr;   // pinned copy of resblk
+  size_t h_res_cap = 0;                  // words
   void* plans = nullptr;                 // KPlan per pattern of the launch
   size_t plans_bytes = 0;
-  void* pcount = nullptr;  // per pattern: count, jobs
-  size_t pcount_bytes = 0;
   Task* tasks = nullptr;
   uint64_t task_cap = 0;
   void* tables = nullptr;  // fold / weight tables
@@ -1837,8 +1843,6 @@ int ensure_workspace(int device) {
   HIP_OK(hipSetDevice(device));
   HIP_OK(hipStreamCreateWithFlags(&g_ws.stream, hipStreamNonBlocking));
   HIP_OK(hipStreamCreateWithFlags(&g_ws.copy_stream, hipStreamNonBlocking));
-  HIP_OK(hipMalloc(&g_ws.cursor, 8 * sizeof(unsigned long long)));
-  HIP_OK(hipMalloc(&g_ws.out, 8 * sizeof(unsigned long long)));
   HIP_OK(hipEventCreate(&g_ws.ev0));
   HIP_OK(hipEventCreate(&g_ws.ev1));
   HIP_OK(hipEventCreate(&g_ws.evf));
@@ -1848,6 +1852,30 @@ int ensure_workspace(int device) {
   g_ws.task_cap = 1ull << 22;  // 4M range tasks (96 MiB)
   HIP_OK(hipMalloc(&g_ws.tasks, sizeof(Task) * g_ws.task_cap));
   g_ws.device = device;
+  return PM_OK;
+}
+
+// the result block for npat patterns (device and pinned host copy)
+int grow_results(int npat) {
+  const size_t words = 16 + 2 * (size_t)std::max(npat, 1);
+  const size_t need = sizeof(unsigned long long) * words;
+  if (need > g_ws.resblk_bytes) {
+    if (g_ws.resblk) HIP_OK(hipFree(g_ws.resblk));
+    g_ws.resblk = nullptr;
+    g_ws.resblk_bytes = 0;
+    HIP_OK(hipMalloc(&g_ws.resblk, need));
+    g_ws.resblk_bytes = need;
+  }
+  if (words > g_ws.h_res_cap) {
+    if (g_ws.h_res) HIP_OK(hipHostFree(g_ws.h_res));
+    g_ws.h_res = nullptr;
+    g_ws.h_res_cap = 0;
+    HIP_OK(hipHostMalloc((void**)&g_ws.h_res, need, hipHostMallocDefault));
+    g_ws.h_res_cap = words;
+  }
+  g_ws.cursor = (unsigned long long*)g_ws.resblk;
+  g_ws.out = g_ws.cursor + 8;
+  g_ws.pcount = g_ws.cursor + 16;
   return PM_OK;
 }
 
@@ -2694,7 +2722,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   }
   rc = grow(&g_ws.plans, &g_ws.plans_bytes, sizeof(KPlan) * npat);
   if (rc) return rc;
-  rc = grow(&g_ws.pcount, &g_ws.pcount_bytes, sizeof(unsigned long long) * 2 * npat);
+  rc = grow_results(npat);
   if (rc) return rc;
   HIP_OK(hipMemcpyAsync(g_ws.plans, kps.data(), sizeof(KPlan) * npat, hipMemcpyHostToDevice, st));
   unsigned long long* d_per_edge = nullptr;
@@ -2737,7 +2765,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   L.nchunk = nchunk;
   L.npos = npos;
   L.sample = sample;
-  L.pcount = (unsigned long long*)g_ws.pcount;
+  L.pcount = g_ws.pcount;
   L.cursor = g_ws.cursor;
   L.edge_lo = lo;
   L.edge_hi = hi;
@@ -2785,9 +2813,8 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       1, std::min<uint64_t>((h.m + kFoldChunk - 1) / kFoldChunk, (uint64_t)g_ws.cus * 16));
   const uint32_t np = paged ? pg->layout.count() : 0;
   for (int r = 0; r < reps; r++) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor, 8, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.out, 8, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, L.pcount, 2ull * npat, 0ull);
+    // cursor, out and pcount in one fill
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, g_ws.cursor, 16 + 2ull * npat, 0ull);
     L.per_edge = (r == 0) ? d_per_edge : nullptr;
     L.root_work = (r == 0) ? d_root_work : nullptr;
     if (L.per_edge)
@@ -2890,11 +2917,8 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     sum_ms += t;
     min_ms = std::min(min_ms, t);
   }
-  unsigned long long out[8], cur[8];
-  std::vector<unsigned long long> pc(2 * (size_t)npat);
-  HIP_OK(hipMemcpyAsync(out, g_ws.out, sizeof(out), hipMemcpyDeviceToHost, st));
-  HIP_OK(hipMemcpyAsync(cur, g_ws.cursor, sizeof(cur), hipMemcpyDeviceToHost, st));
-  HIP_OK(hipMemcpyAsync(pc.data(), g_ws.pcount, sizeof(unsigned long long) * pc.size(),
+  // cursor, out and pcount in one copy into pinned memory
+  HIP_OK(hipMemcpyAsync(g_ws.h_res, g_ws.resblk, sizeof(unsigned long long) * (16 + 2 * (size_t)npat),
                         hipMemcpyDeviceToHost, st));
   if (d_per_edge) {
     HIP_OK(hipMemcpyAsync(opts->per_edge + lo, d_per_edge, sizeof(unsigned long long) * (hi - lo),
@@ -2907,6 +2931,9 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     HIP_OK(hipFreeAsync(d_root_work, st));
   }
   HIP_OK(hipStreamSynchronize(st));
+  const unsigned long long* cur = g_ws.h_res;
+  const unsigned long long* out = g_ws.h_res + 8;
+  const unsigned long long* pc = g_ws.h_res + 16;
   if (opts->package_use) {
     // per vertex: the rounds in which its package was used (0 when resident)
     for (uint64_t v = 0; v < h.n; v++)
