@@ -116,6 +116,9 @@ def batched_section(pm):
     rs = pm.match_batch(g, pats, repeats=3, **opts)
     assert [r.count for r in rs] == plain, "batched counts differ from single-pattern runs"
     batch_ms = rs[0].min_ms
+    # the same launch as a captured hipGraph, replayed
+    rg = pm.match_batch(g, pats, repeats=3, graph=True, **opts)
+    assert [r.count for r in rg] == plain, "graph replays differ from single-pattern runs"
     return {
         "workload": ("rmat14-ef16-labeled8: RMAT scale 14, edge factor 16, seed 5, 8 labels; "
                      "64 random connected labeled patterns, 3-7 vertices, density 0.3-0.8, seed 6"),
@@ -125,6 +128,7 @@ def batched_section(pm):
         "plain_ms_per_pattern": round(plain_ms / len(pats), 4),
         "batched_tuned_ms": round(batch_ms, 3),
         "batched_ms_per_pattern": round(batch_ms / len(pats), 4),
+        "batched_graph_replay_ms": round(rg[0].min_ms, 3),
         "speedup": round(plain_ms / batch_ms, 3),
         "counts_equal": True,
         "autotune_s": round(tune_s, 2),
@@ -328,6 +332,12 @@ def main():
     counts = {r.count for r in results} | {first.count}
     assert len(counts) == 1, f"count changed between steps: {counts}"
     count = first.count
+    # the same steps as captured-hipGraph replays (first call captures)
+    pm.match(g, p, graph=True, **opts)
+    t0 = time.perf_counter()
+    graph_runs = [pm.match(g, p, graph=True, **opts) for _ in range(args.steps)]
+    graph_ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    assert all(r.count == first.count for r in graph_runs)
     untuned = [pm.match(g, p) for _ in range(3)]
     assert all(r.count == count for r in untuned)
     untuned_ms = float(np.mean([r.kernel_ms for r in untuned]))
@@ -397,6 +407,7 @@ def main():
         "count": count,
         "jobs": first.jobs,
         "kernel_ms": round(kernel_ms, 4),
+        "graph_replay_ms_per_step": round(graph_ms_per_step, 4),
         "untuned_kernel_ms": round(untuned_ms, 4),
         "tuned_options": opts,
         "autotune": tune,

@@ -87,6 +87,12 @@ typedef struct {
                            managed memory (src/config.h:41,
                            src/gpu_match.cu:478,601,663,752).  0 = 1024 (the
                            reference's value), < 0 = off */
+  int32_t graph;        /* 1: run the launch sequence (result fill, pendant
+                           tables, match kernels) as one hipGraph, captured on
+                           first use and replayed by later runs and calls
+                           whose kernel arguments are all unchanged.  Rows
+                           resident, no per_edge / traces (else ignored);
+                           phase_ms stays 0 */
 } pm_opts;
 
 #define PM_LEARN_ORDER 0x1  /* phase-1 jobs heaviest predicted root first */

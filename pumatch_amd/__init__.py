@@ -56,6 +56,7 @@ class Opts(C.Structure):
         ("root_work", C.POINTER(C.c_uint64)),
         ("package_use", C.POINTER(C.c_uint32)),
         ("direct_degree", C.c_int32),
+        ("graph", C.c_int32),
     ]
 
 
@@ -430,7 +431,7 @@ POLICY = {"lru": 0, "fifo": 1, "learned": 2}
 
 def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
                stream, cache_bytes, package_bytes, policy, prefetch, order="cost", sample=0,
-               model=None, learned=0, direct_degree=0):
+               model=None, learned=0, direct_degree=0, graph=False):
     o = Opts()
     lib().pm_opts_default(C.byref(o))
     o.waves, o.chunk, o.repeats = waves, chunk, repeats
@@ -441,6 +442,7 @@ def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge
     o.order = ORDER[order]
     o.sample = sample
     o.direct_degree = direct_degree
+    o.graph = 1 if graph else 0
     if model is not None:
         o.model = model._h.value
         o.learned = learned if learned else (PM_LEARN_ORDER | PM_LEARN_PAGING)
@@ -466,7 +468,7 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
           per_edge: bool = False, stream: int | None = None, cache_bytes: int = 0,
           package_bytes: int = 0, policy: str = "lru", prefetch: bool = False,
           order: str = "cost", sample: int = 0, model=None, learned: int = 0,
-          traces: bool = False, direct_degree: int = 0) -> MatchResult:
+          traces: bool = False, direct_degree: int = 0, graph: bool = False) -> MatchResult:
     """Count matches of p in g on the GPU (pm_match).  order="file" matches
     labeled patterns in file order as the reference does; "cost" (default)
     lets the planner pick the order of vertices 2.. (same count).  With a
@@ -474,10 +476,12 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
     order and package ranking.  traces=True records root_work and
     package_use (both per vertex) into the result.  direct_degree (out of
     core): missing rows of at most that many entries are read from pinned
-    host memory instead of being deferred (0: 1024, < 0: never)."""
+    host memory instead of being deferred (0: 1024, < 0: never).  graph=True
+    replays the run's launch sequence as a captured hipGraph (resident rows;
+    phase_ms is then not measured)."""
     o, pe = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
                        stream, cache_bytes, package_bytes, policy, prefetch, order, sample,
-                       model, learned, direct_degree)
+                       model, learned, direct_degree, graph)
     rw = pu = None
     if traces:
         rw = np.zeros(max(g.n, 1), dtype=np.uint64)
@@ -497,7 +501,8 @@ def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: 
                 split: int = 0, hub_factor: int = 0, edge_lo: int = 0, edge_hi: int = 0,
                 stream: int | None = None, cache_bytes: int = 0, package_bytes: int = 0,
                 policy: str = "lru", prefetch: bool = False, order: str = "cost",
-                sample: int = 0, model=None, learned: int = 0, direct_degree: int = 0) -> list:
+                sample: int = 0, model=None, learned: int = 0, direct_degree: int = 0,
+                graph: bool = False) -> list:
     """Count several patterns in one persistent launch (pm_match_batch).
     Returns one MatchResult per pattern; count/jobs are the pattern's own,
     the timing fields describe the whole launch."""
@@ -505,7 +510,7 @@ def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: 
     n = len(patterns)
     o, _ = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, False,
                       stream, cache_bytes, package_bytes, policy, prefetch, order, sample,
-                      model, learned, direct_degree)
+                      model, learned, direct_degree, graph)
     hs = (C.c_void_p * max(n, 1))(*[q._h.value for q in patterns])
     rs = (Result * max(n, 1))()
     _check(lib().pm_match_batch(g._h, hs, n, C.byref(o), rs), "match_batch")

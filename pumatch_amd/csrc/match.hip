@@ -1832,6 +1832,10 @@ struct Workspace {
   uint8_t* h_used = nullptr;             // pinned per-slot use of a round
   size_t h_used_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, eve = nullptr, evc = nullptr;
+  // captured launch sequence of the last graph-mode run and the bytes of
+  // every argument it was captured with
+  hipGraphExec_t gexec = nullptr;
+  std::string gsig;
   int cus = 0;
   int resident = 0;
 };
@@ -2812,7 +2816,96 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   const int fold_blocks = (int)std::max<uint64_t>(
       1, std::min<uint64_t>((h.m + kFoldChunk - 1) / kFoldChunk, (uint64_t)g_ws.cus * 16));
   const uint32_t np = paged ? pg->layout.count() : 0;
-  for (int r = 0; r < reps; r++) {
+  // pendant-tree tables, children first (table ids are in that order)
+  struct FoldStep {
+    int kind;  // 0 product, 1 leaf over label-grouped rows, 2 fill + fold
+    FoldArgs A;
+  };
+  std::vector<FoldStep> folds;
+  if (h.n)
+    for (const TableJob& j : table_jobs) {
+      const Plan& pl = *plans[j.plan];
+      const int id = (int)(&j - table_jobs.data());
+      // built in place after zeroing, so padding bytes are zero too (the
+      // bytes take part in the graph signature)
+      folds.emplace_back();
+      FoldStep& f = folds.back();
+      std::memset(&f, 0, sizeof(f));
+      FoldArgs& A = f.A;
+      A.g = kg;
+      A.want = (uint32_t)pl.vlabel[j.u];
+      for (int c : j.child) A.child[A.nchild++] = table(c);
+      A.out = table(id);
+      A.bytes = g_ws.out + 6;
+      f.kind = j.product ? 0 : (A.nchild == 0 && !paged && kg.nl > 0 && A.want + 1 < kg.nl) ? 1 : 2;
+    }
+  auto launch_folds = [&]() {
+    for (const FoldStep& f : folds) {
+      if (f.kind == 0) {
+        hipLaunchKernelGGL(k_product, dim3(1024), dim3(256), 0, st, h.n, f.A);
+      } else if (f.kind == 1) {
+        // a leaf with label-grouped rows: the label part's size, no row reads
+        hipLaunchKernelGGL(k_fold_leaf, dim3((unsigned)std::min<uint64_t>((h.n + 255) / 256, 4096)),
+                           dim3(256), 0, st, f.A);
+      } else {
+        hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, st, f.A.out, h.n, 0ull);
+        if (h.m) hipLaunchKernelGGL(k_fold, dim3(fold_blocks), dim3(kWave), 0, st, f.A);
+      }
+    }
+  };
+  // Graph mode (resident rows, no per-edge counts or traces): the whole
+  // launch sequence of a run — result fill, tables, match kernels — is one
+  // hipGraph, captured once and replayed while every argument is unchanged.
+  const bool use_graph = opts->graph && !paged && !d_per_edge && !d_root_work;
+  if (use_graph) {
+    L.rout = (Resume*)g_ws.resume[1];
+    std::string sig;
+    auto put = [&](const void* p, size_t n) { sig.append((const char*)p, n); };
+    put(&L, sizeof(L));
+    for (const FoldStep& f : folds) put(&f, sizeof(f));
+    put(&waves, sizeof(waves));
+    put(&fold_blocks, sizeof(fold_blocks));
+    put(&npat, sizeof(npat));
+    put(&st, sizeof(st));
+    if (!g_ws.gexec || sig != g_ws.gsig) {
+      if (g_ws.gexec) HIP_OK(hipGraphExecDestroy(g_ws.gexec));
+      g_ws.gexec = nullptr;
+      g_ws.gsig.clear();
+      HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, g_ws.cursor, 16 + 2ull * npat, 0ull);
+      launch_folds();
+      hipLaunchKernelGGL(k_match_edges<false>, dim3(waves), dim3(kWave), 0, st, L);
+      if (L.tasks) hipLaunchKernelGGL(k_match_tasks<false>, dim3(waves), dim3(kWave), 0, st, L);
+      const hipError_t le = hipGetLastError();
+      hipGraph_t graph = nullptr;
+      const hipError_t ce = hipStreamEndCapture(st, &graph);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        set_error(std::string("graph capture: ") +
+                  hipGetErrorString(le != hipSuccess ? le : ce));
+        return PM_ERR_DEVICE;
+      }
+      const hipError_t ie = hipGraphInstantiate(&g_ws.gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        g_ws.gexec = nullptr;
+        set_error(std::string("graph instantiate: ") + hipGetErrorString(ie));
+        return PM_ERR_DEVICE;
+      }
+      g_ws.gsig = sig;
+    }
+  }
+  for (int r = 0; r < reps && use_graph; r++) {
+    HIP_OK(hipEventRecord(g_ws.ev0, st));
+    HIP_OK(hipGraphLaunch(g_ws.gexec, st));
+    HIP_OK(hipEventRecord(g_ws.ev1, st));
+    HIP_OK(hipEventSynchronize(g_ws.ev1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, g_ws.ev0, g_ws.ev1));
+    sum_ms += ms;
+    min_ms = std::min(min_ms, (double)ms);
+  }
+  for (int r = 0; r < reps && !use_graph; r++) {
     // cursor, out and pcount in one fill
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, g_ws.cursor, 16 + 2ull * npat, 0ull);
     L.per_edge = (r == 0) ? d_per_edge : nullptr;
@@ -2861,29 +2954,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
                          (pg->nslots + 3) / 4, 0u);
     }
     HIP_OK(hipEventRecord(g_ws.ev0, st));
-    // pendant-tree tables, children first (table ids are in that order)
-    for (const TableJob& j : table_jobs) {
-      const Plan& pl = *plans[j.plan];
-      const int id = (int)(&j - table_jobs.data());
-      FoldArgs A;
-      std::memset(&A, 0, sizeof(A));
-      A.g = kg;
-      A.want = (uint32_t)pl.vlabel[j.u];
-      for (int c : j.child) A.child[A.nchild++] = table(c);
-      A.out = table(id);
-      A.bytes = g_ws.out + 6;
-      if (!h.n) continue;
-      if (j.product) {
-        hipLaunchKernelGGL(k_product, dim3(1024), dim3(256), 0, st, h.n, A);
-      } else if (A.nchild == 0 && !paged && kg.nl > 0 && A.want + 1 < kg.nl) {
-        // a leaf with label-grouped rows: the label part's size, no row reads
-        hipLaunchKernelGGL(k_fold_leaf, dim3((unsigned)std::min<uint64_t>((h.n + 255) / 256, 4096)),
-                           dim3(256), 0, st, A);
-      } else {
-        hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, st, A.out, h.n, 0ull);
-        if (h.m) hipLaunchKernelGGL(k_fold, dim3(fold_blocks), dim3(kWave), 0, st, A);
-      }
-    }
+    launch_folds();
     HIP_OK(hipEventRecord(g_ws.evf, st));
     L.rout = (Resume*)g_ws.resume[1];
     hipLaunchKernelGGL(paged ? k_match_edges<true> : k_match_edges<false>, dim3(waves), dim3(kWave), 0, st, L);

@@ -47,6 +47,8 @@ def test_random_options_match_oracle(seed):
             kw["hub_factor"] = int(rng.choice([1, 2, 16]))
         if rng.random() < 0.3:
             kw["order"] = "file"
+        if rng.random() < 0.4:
+            kw["graph"] = True
         print(f"[stress] seed={seed} trial={trial} {kw}", flush=True)
         got = [r.count for r in pm.match_batch(g, pats, **kw)]
         assert got == want, kw
