@@ -2128,6 +2128,10 @@ struct PagedGraph {
   std::vector<CopyJob> jobs;       // host staging of the same
   std::vector<int32_t> pkg_slot;  // host mirror
   CachePolicy policy;
+  // per package: entries of its rows longer than `long_for` (the rows that
+  // can miss when shorter ones are read directly); computed on demand
+  std::vector<uint64_t> long_words;
+  uint32_t long_for = ~0u;
 };
 
 std::mutex g_paged_mu;
@@ -2926,15 +2930,49 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       std::vector<uint8_t> pinned(pg->nslots, 0);
       std::vector<uint32_t> pre;
       uint64_t used_slots = 0;
-      // learned prefetch preloads the packages of highest predicted use
-      for (uint32_t i = 0; i < np; i++) {
-        const uint32_t p = (opts->prefetch && !by_score.empty()) ? by_score[i] : i;
-        if (used_slots + pg->layout.nslots[p] > pg->nslots) {
-          if (by_score.empty() || !opts->prefetch) break;
-          continue;
+      const bool learned_pre = opts->prefetch && !by_score.empty();
+      if (kg.direct_max > 0 && !learned_pre) {
+        // Rows of <= direct_max entries are read directly and never miss,
+        // so packages holding longer rows go first, longest first; the
+        // remaining slots take the other packages in id order (a cached
+        // short row is still faster than a read over the link)
+        if (pg->long_for != kg.direct_max) {
+          pg->long_words.assign(np, 0);
+          for (uint64_t v = 0; v < h.n; v++) {
+            const uint64_t d = h.rowptr[v + 1] - h.rowptr[v];
+            if (d > kg.direct_max) pg->long_words[pg->layout.pkg_of[v]] += d;
+          }
+          pg->long_for = kg.direct_max;
         }
-        pre.push_back(p);
-        used_slots += pg->layout.nslots[p];
+        std::vector<uint32_t> order;
+        for (uint32_t p = 0; p < np; p++)
+          if (pg->long_words[p]) order.push_back(p);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+          return pg->long_words[a] > pg->long_words[b];
+        });
+        for (uint32_t p : order) {
+          if (used_slots + pg->layout.nslots[p] > pg->nslots) continue;
+          pre.push_back(p);
+          used_slots += pg->layout.nslots[p];
+        }
+        for (uint32_t p = 0; p < np; p++) {
+          if (pg->long_words[p]) continue;
+          if (used_slots + pg->layout.nslots[p] > pg->nslots) break;
+          pre.push_back(p);
+          used_slots += pg->layout.nslots[p];
+        }
+      } else {
+        // learned prefetch preloads the packages of highest predicted use;
+        // otherwise packages in id order
+        for (uint32_t i = 0; i < np; i++) {
+          const uint32_t p = learned_pre ? by_score[i] : i;
+          if (used_slots + pg->layout.nslots[p] > pg->nslots) {
+            if (!learned_pre) break;
+            continue;
+          }
+          pre.push_back(p);
+          used_slots += pg->layout.nslots[p];
+        }
       }
       HIP_OK(hipEventRecord(g_ws.evc, g_ws.copy_stream));
       uint64_t b = 0;
