@@ -2931,11 +2931,12 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       std::vector<uint32_t> pre;
       uint64_t used_slots = 0;
       const bool learned_pre = opts->prefetch && !by_score.empty();
-      if (kg.direct_max > 0 && !learned_pre) {
+      if (kg.direct_max > 0) {
         // Rows of <= direct_max entries are read directly and never miss,
         // so packages holding longer rows go first, longest first; the
-        // remaining slots take the other packages in id order (a cached
-        // short row is still faster than a read over the link)
+        // remaining slots take the other packages in id order, or by
+        // predicted use with learned prefetch (a cached short row is still
+        // faster than a read over the link)
         if (pg->long_for != kg.direct_max) {
           pg->long_words.assign(np, 0);
           for (uint64_t v = 0; v < h.n; v++) {
@@ -2955,9 +2956,13 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
           pre.push_back(p);
           used_slots += pg->layout.nslots[p];
         }
-        for (uint32_t p = 0; p < np; p++) {
+        for (uint32_t i = 0; i < np; i++) {
+          const uint32_t p = learned_pre ? by_score[i] : i;
           if (pg->long_words[p]) continue;
-          if (used_slots + pg->layout.nslots[p] > pg->nslots) break;
+          if (used_slots + pg->layout.nslots[p] > pg->nslots) {
+            if (!learned_pre) break;
+            continue;
+          }
           pre.push_back(p);
           used_slots += pg->layout.nslots[p];
         }
@@ -2976,10 +2981,13 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       }
       HIP_OK(hipEventRecord(g_ws.evc, g_ws.copy_stream));
       uint64_t b = 0;
+      const auto c0 = std::chrono::steady_clock::now();
       rc = load_packages(pg, pre, pinned, g_ws.copy_stream, &b);
       if (rc) return rc;
       rc = flush_packages(pg, g_ws.copy_stream);
       if (rc) return rc;
+      link_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0)
+                     .count();
       link_bytes += b;
       HIP_OK(hipMemcpyAsync(pg->d_pkg_slot, pg->pkg_slot.data(), sizeof(int32_t) * np,
                             hipMemcpyHostToDevice, g_ws.copy_stream));
