@@ -1610,7 +1610,11 @@ struct FoldArgs {
   unsigned long long* bytes;  // algorithmic bytes accumulator
 };
 
-constexpr uint32_t kFoldChunk = 512;
+// edges per wave of a fold pass (build-time tunable)
+#ifndef PM_FOLD_CHUNK
+#define PM_FOLD_CHUNK 1024
+#endif
+constexpr uint32_t kFoldChunk = PM_FOLD_CHUNK;
 
 __global__ __launch_bounds__(64) void k_fold(FoldArgs A) {
   const KGraph& g = A.g;
@@ -2818,7 +2822,8 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   uint64_t link_bytes = 0, misses = 0, deferred = 0;
   int rounds = 1;
   const int fold_blocks = (int)std::max<uint64_t>(
-      1, std::min<uint64_t>((h.m + kFoldChunk - 1) / kFoldChunk, (uint64_t)g_ws.cus * 16));
+      1, std::min<uint64_t>((h.m + kFoldChunk - 1) / kFoldChunk,
+                            (uint64_t)g_ws.cus * 16 * std::max<uint32_t>(1, 512 / kFoldChunk)));
   const uint32_t np = paged ? pg->layout.count() : 0;
   // pendant-tree tables, children first (table ids are in that order)
   struct FoldStep {
