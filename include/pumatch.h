@@ -93,6 +93,11 @@ typedef struct {
                            whose kernel arguments are all unchanged.  Rows
                            resident, no per_edge / traces (else ignored);
                            phase_ms stays 0 */
+  int32_t resplit_us;   /* rows resident: a search that has run longer than
+                           this many microseconds hands its untried
+                           candidates back as pieces, which further rounds
+                           spread over all waves (work stealing, as
+                           src/gpu_match.cu:21-148,818-853).  <= 0 = off */
 } pm_opts;
 
 #define PM_LEARN_ORDER 0x1  /* phase-1 jobs heaviest predicted root first */

@@ -57,6 +57,7 @@ class Opts(C.Structure):
         ("package_use", C.POINTER(C.c_uint32)),
         ("direct_degree", C.c_int32),
         ("graph", C.c_int32),
+        ("resplit_us", C.c_int32),
     ]
 
 
@@ -431,7 +432,7 @@ POLICY = {"lru": 0, "fifo": 1, "learned": 2}
 
 def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
                stream, cache_bytes, package_bytes, policy, prefetch, order="cost", sample=0,
-               model=None, learned=0, direct_degree=0, graph=False):
+               model=None, learned=0, direct_degree=0, graph=False, resplit_us=0):
     o = Opts()
     lib().pm_opts_default(C.byref(o))
     o.waves, o.chunk, o.repeats = waves, chunk, repeats
@@ -443,6 +444,7 @@ def _make_opts(g: Graph, waves, chunk, repeats, split, hub_factor, edge_lo, edge
     o.sample = sample
     o.direct_degree = direct_degree
     o.graph = 1 if graph else 0
+    o.resplit_us = resplit_us
     if model is not None:
         o.model = model._h.value
         o.learned = learned if learned else (PM_LEARN_ORDER | PM_LEARN_PAGING)
@@ -468,7 +470,8 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
           per_edge: bool = False, stream: int | None = None, cache_bytes: int = 0,
           package_bytes: int = 0, policy: str = "lru", prefetch: bool = False,
           order: str = "cost", sample: int = 0, model=None, learned: int = 0,
-          traces: bool = False, direct_degree: int = 0, graph: bool = False) -> MatchResult:
+          traces: bool = False, direct_degree: int = 0, graph: bool = False,
+          resplit_us: int = 0) -> MatchResult:
     """Count matches of p in g on the GPU (pm_match).  order="file" matches
     labeled patterns in file order as the reference does; "cost" (default)
     lets the planner pick the order of vertices 2.. (same count).  With a
@@ -481,7 +484,7 @@ def match(g: Graph, p: Pattern, *, waves: int = 0, chunk: int = 0, repeats: int 
     phase_ms is then not measured)."""
     o, pe = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, per_edge,
                        stream, cache_bytes, package_bytes, policy, prefetch, order, sample,
-                       model, learned, direct_degree, graph)
+                       model, learned, direct_degree, graph, resplit_us)
     rw = pu = None
     if traces:
         rw = np.zeros(max(g.n, 1), dtype=np.uint64)
@@ -502,7 +505,7 @@ def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: 
                 stream: int | None = None, cache_bytes: int = 0, package_bytes: int = 0,
                 policy: str = "lru", prefetch: bool = False, order: str = "cost",
                 sample: int = 0, model=None, learned: int = 0, direct_degree: int = 0,
-                graph: bool = False) -> list:
+                graph: bool = False, resplit_us: int = 0) -> list:
     """Count several patterns in one persistent launch (pm_match_batch).
     Returns one MatchResult per pattern; count/jobs are the pattern's own,
     the timing fields describe the whole launch."""
@@ -510,7 +513,7 @@ def match_batch(g: Graph, patterns, *, waves: int = 0, chunk: int = 0, repeats: 
     n = len(patterns)
     o, _ = _make_opts(g, waves, chunk, repeats, split, hub_factor, edge_lo, edge_hi, False,
                       stream, cache_bytes, package_bytes, policy, prefetch, order, sample,
-                      model, learned, direct_degree, graph)
+                      model, learned, direct_degree, graph, resplit_us)
     hs = (C.c_void_p * max(n, 1))(*[q._h.value for q in patterns])
     rs = (Result * max(n, 1))()
     _check(lib().pm_match_batch(g._h, hs, n, C.byref(o), rs), "match_batch")

@@ -86,6 +86,10 @@ constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
 constexpr int kWalk = PM_FUSED_WALK;
 constexpr uint32_t kResumeClaim = 16;  // deferred pieces claimed per atomic
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+// range-task buffer: 2^PM_TASK_CAP_LOG2 tasks of 32 B (build-time tunable)
+#ifndef PM_TASK_CAP_LOG2
+#define PM_TASK_CAP_LOG2 22
+#endif
 constexpr int kMaxDepth = 6;
 
 struct KOp {
@@ -175,6 +179,7 @@ struct DeferOut {
   unsigned long long* count;
   uint64_t cap;
   uint64_t soft;  // beyond this many pieces, searches stop at their first miss
+  uint64_t budget;  // resident: wall-clock ticks after which a search re-splits (0 = never)
 };
 
 struct WaveState {
@@ -208,6 +213,8 @@ struct WaveState {
   uint32_t jlo, jhi;
   uint32_t pat;           // its pattern (batched launches)
   unsigned long long acc_count, acc_jobs;  // phase 1: totals of pattern `pat`
+  unsigned long long t0;  // wall clock at the start of the current search
+  int32_t rs_depth, rs_stop;  // pending re-split (depth, first open depth); -1: none
   uint32_t blk;     // package of the last row found missing
   int32_t rdepth;   // resume input: depth, candidate indices, range end
   uint32_t ridx[kMaxDepth];
@@ -743,6 +750,29 @@ __device__ void emit_one(const DeferOut& O, const WaveState& ws, int depth, uint
   }
 }
 
+// A resident search stopped by its re-split budget (run_job recorded
+// rs_depth / rs_stop, right after the subtree of candidate cidx[rs_depth]
+// was finished): the rest of that depth goes out as up to 64 pieces, the
+// rest of each shallower open depth as one piece.  Called by the kernels
+// after run_job returns, outside the search loop's register pressure.
+template <bool PG, bool RS>
+__device__ __forceinline__ void resplit_flush(const DeferOut& O, WaveState& ws, uint32_t lane) {
+  if (PG || !RS) return;
+  const int dd = (int)uni((uint32_t)ws.rs_depth);
+  if (dd < 0) return;
+  const uint32_t a0 = uni(ws.cidx[dd]) + 1, b0 = uni(ws.clen[dd]);
+  if (a0 < b0) {
+    const uint32_t np = min(b0 - a0, 64u), per = (b0 - a0 + np - 1) / np;
+    for (uint32_t a = a0; a < b0; a += per) emit_one(O, ws, dd, a, min(a + per, b0), 0, lane);
+  }
+  for (int d = (int)uni((uint32_t)ws.rs_stop); d < dd; d++) {
+    const uint32_t s0 = uni(ws.cidx[d]) + 1, s1 = uni(ws.clen[d]);
+    if (s0 < s1) emit_one(O, ws, d, s0, s1, 0, lane);
+  }
+  if (lane == 0) ws.rs_depth = -1;
+  __syncthreads();
+}
+
 // Lane-parallel emission: every lane with `mine` writes the piece "candidate
 // i at this depth" (range [i, i+1)), stopped by package blk.
 __device__ void emit_lanes(const DeferOut& O, const WaveState& ws, int depth, bool mine,
@@ -1083,8 +1113,9 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
 // subtree that needs a non-resident row is written to O as a deferred piece
 // and the search goes on with the next candidate, so each piece is searched
 // exactly once.  Phase 1 (may_split) may hand a tail of the range to the
-// task buffer.
-template <bool PG>
+// task buffer.  RS: resident re-split compiled in (only kernels of batches
+// that can use it carry its registers).
+template <bool PG, bool RS = false>
 __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState& ws,
                                       uint32_t* arena, uint32_t cap, uint32_t lane, uint32_t r,
                                       uint32_t c, uint64_t e, uint32_t lo, uint32_t hi,
@@ -1092,6 +1123,9 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
                                       unsigned long long* ntasks, uint64_t task_cap,
                                       const DeferOut& O, Counters& ct) {
   const int k = P.k;
+  // resident re-split: a search running past its budget hands its untried
+  // candidates back as pieces (below)
+  if (RS && !PG && O.budget && lane == 0) ws.t0 = wall_clock64();
   ws.je = e;
   ws.jlo = lo;
   ws.jhi = hi;
@@ -1247,6 +1281,23 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
       if (a0 < b0) emit_one(O, ws, dd, a0, b0, uni(ws.blk), lane);
     }
   };
+  // Resident re-split (work stealing, src/gpu_match.cu:21-148 splits a
+  // victim's shallow level the same way): called right after the subtree of
+  // candidate cidx[d] at depth d is finished, so every search makes
+  // progress before it splits.  Past the budget, the rest of depth d goes
+  // out as up to 64 pieces and the rest of the shallower depths as one
+  // piece each; later rounds resume them over all waves.
+  auto resplit = [&](int dd) -> bool {
+    if (PG || !RS || O.budget == 0) return false;
+    if (!uni((uint32_t)(wall_clock64() - uni64(ws.t0) > O.budget))) return false;
+    if (uni64(*(volatile unsigned long long*)O.count) >= O.soft) return false;
+    // the pieces are written by resplit_flush once the search has returned
+    if (lane == 0) {
+      ws.rs_depth = dd;
+      ws.rs_stop = stop;
+    }
+    return true;
+  };
   for (;;) {
     if (fused_now) {
       fused_now = false;
@@ -1258,12 +1309,14 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
       }
       if (d == stop) break;
       d--;
+      if (resplit(d)) return cnt;
       continue;
     }
     uint32_t x, ix;
     if (!next_cand<PG>(g, P, ws, d, lane, x, ix, ct)) {
       if (d == stop) break;
       d--;
+      if (resplit(d)) return cnt;
       continue;
     }
     ws.cidx[d] = ix;
@@ -1360,6 +1413,9 @@ struct Launch {
   Resume* rout;  // deferred pieces for the next round
   uint64_t rout_cap;
   uint64_t soft;  // beyond this many new pieces, searches stop at their first miss
+  uint64_t budget;  // resident re-split budget in wall-clock ticks (0 = off)
+  uint32_t claim;   // pieces claimed per atomic by k_resume (0 = kResumeClaim)
+  uint32_t pad3;
   // learned job order: position s lies in the row of root_order[i] where
   // root_start[i] <= s < root_start[i+1] (null: CSR order)
   const uint32_t* root_order;
@@ -1379,10 +1435,11 @@ __device__ __forceinline__ DeferOut defer_out(const Launch& L) {
   O.count = &L.cursor[4];
   O.cap = L.rout_cap;
   O.soft = L.soft;
+  O.budget = L.budget;
   return O;
 }
 
-template <bool PG>
+template <bool PG, bool RS = false>
 __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_edges(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
@@ -1395,6 +1452,7 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_edges(
     ws.pat = 0xFFFFFFFFu;
     ws.acc_count = ws.acc_jobs = 0;
     ws.lcount = 0;
+    ws.rs_depth = -1;
   }
   __syncthreads();
   const DeferOut O = defer_out(L);
@@ -1458,8 +1516,9 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_edges(
         const uint64_t ej = uni64(ws.jedge[i]);
         if (lane == 0) ws.alg0 = ct.alg;
         const unsigned long long cnt =
-            run_job<PG>(g, P, ws, arena, L.cap, lane, jr, jc, ej, 0, kInf, false,
-                        L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
+            run_job<PG, RS>(g, P, ws, arena, L.cap, lane, jr, jc, ej, 0, kInf, false,
+                            L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
+        resplit_flush<PG, RS>(O, ws, lane);
         if (L.root_work && lane == 0)
           atomicAdd(&L.root_work[jr], (unsigned long long)(ct.alg - ws.alg0));
         if (lane == 0) {
@@ -1482,7 +1541,7 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_edges(
 }
 
 // Range tasks (phase 2).
-template <bool PG>
+template <bool PG, bool RS = false>
 __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_tasks(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
@@ -1490,7 +1549,10 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_tasks(
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
   Counters ct;
   const DeferOut O = defer_out(L);
-  if (lane == 0) ws.lcount = 0;
+  if (lane == 0) {
+    ws.lcount = 0;
+    ws.rs_depth = -1;
+  }
   __syncthreads();
   const uint64_t ntask = L.tasks ? min((uint64_t)L.cursor[2], L.task_cap) : 0;
   for (;;) {
@@ -1510,8 +1572,9 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_tasks(
     ws.pat = p;
     if (lane == 0) ws.alg0 = ct.alg;
     const unsigned long long cnt =
-        run_job<PG>(g, L.plans[p], ws, arena, L.cap, lane, r, c, e, tlo, thi, false, false,
-                    nullptr, nullptr, 0, O, ct);
+        run_job<PG, RS>(g, L.plans[p], ws, arena, L.cap, lane, r, c, e, tlo, thi, false, false,
+                        nullptr, nullptr, 0, O, ct);
+    resplit_flush<PG, RS>(O, ws, lane);
     if (L.root_work && lane == 0)
       atomicAdd(&L.root_work[r], (unsigned long long)(ct.alg - ws.alg0));
     if (lane == 0 && cnt) atomicAdd(&L.pcount[2 * p], cnt);
@@ -1524,7 +1587,7 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_match_tasks(
 }
 
 // Resume round: the admitted deferred pieces (a depth-0 piece may split).
-template <bool PG>
+template <bool PG, bool RS = false>
 __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_resume(Launch L) {
   __shared__ WaveState ws;
   const KGraph& g = L.g;
@@ -1532,7 +1595,10 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_resume(Launc
   uint32_t* arena = L.arena + (size_t)blockIdx.x * L.arena_stride;
   Counters ct;
   const DeferOut O = defer_out(L);
-  if (lane == 0) ws.lcount = 0;
+  if (lane == 0) {
+    ws.lcount = 0;
+    ws.rs_depth = -1;
+  }
   __syncthreads();
   const uint64_t nrin = L.nrin_dev ? uni64(*L.nrin_dev) : L.nrin;
   // Pieces are claimed kResumeClaim at a time.  Consecutive one-candidate
@@ -1540,11 +1606,12 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_resume(Launc
   // batch of candidates deferred) run as one group: the path is rebuilt
   // once and the fused step walks all their candidates together.
   for (;;) {
+    const uint32_t claim = L.claim ? L.claim : kResumeClaim;
     unsigned long long t0 = 0;
-    if (lane == 0) t0 = atomicAdd(&L.cursor[3], (unsigned long long)kResumeClaim);
+    if (lane == 0) t0 = atomicAdd(&L.cursor[3], (unsigned long long)claim);
     t0 = uni64(__shfl(t0, 0));
     if (t0 >= nrin) break;
-    const uint64_t t1 = min(t0 + kResumeClaim, nrin);
+    const uint64_t t1 = min(t0 + claim, nrin);
     for (uint64_t t = t0; t < t1;) {
       const Resume& R0 = L.rin[t];
       const uint64_t e = uni64(R0.edge);
@@ -1578,8 +1645,10 @@ __global__ __launch_bounds__(64, PG ? 4 : PM_RESIDENT_WAVES) void k_resume(Launc
       }
       __syncthreads();
       const unsigned long long cnt =
-          run_job<PG>(g, P, ws, arena, L.cap, lane, r, c, e, rlo, rhi, true,
-                      depth == 0 && L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O, ct);
+          run_job<PG, RS>(g, P, ws, arena, L.cap, lane, r, c, e, rlo, rhi, true,
+                          depth == 0 && L.tasks != nullptr, L.tasks, &L.cursor[2], L.task_cap, O,
+                          ct);
+      resplit_flush<PG, RS>(O, ws, lane);
       if (lane == 0) ws.lcount = 0;
       __syncthreads();
       if (L.root_work && lane == 0)
@@ -1840,6 +1909,7 @@ struct Workspace {
   // every argument it was captured with
   hipGraphExec_t gexec = nullptr;
   std::string gsig;
+  int wall_khz = 0;  // rate of wall_clock64()
   int cus = 0;
   int resident = 0;
 };
@@ -1857,7 +1927,8 @@ int ensure_workspace(int device) {
   HIP_OK(hipEventCreate(&g_ws.eve));
   HIP_OK(hipEventCreate(&g_ws.evc));
   HIP_OK(hipDeviceGetAttribute(&g_ws.cus, hipDeviceAttributeMultiprocessorCount, device));
-  g_ws.task_cap = 1ull << 22;  // 4M range tasks (96 MiB)
+  HIP_OK(hipDeviceGetAttribute(&g_ws.wall_khz, hipDeviceAttributeWallClockRate, device));
+  g_ws.task_cap = 1ull << PM_TASK_CAP_LOG2;  // range tasks (32 B each)
   HIP_OK(hipMalloc(&g_ws.tasks, sizeof(Task) * g_ws.task_cap));
   g_ws.device = device;
   return PM_OK;
@@ -2553,6 +2624,42 @@ int run_rounds(PagedGraph* pg, Launch& L, hipStream_t st, int waves, uint64_t sp
   return PM_OK;
 }
 
+// Resident runs with re-split on: resume the pieces the last launch wrote
+// (in resume[1]) over all waves, round after round, until a round writes
+// none.  One wait per round, for the piece count.
+int resident_rounds(Launch& L, hipStream_t st, int waves, uint64_t rcap, int* rounds) {
+  int b = 1;
+  for (;;) {
+    HIP_OK(hipMemcpyAsync(g_ws.h_res, g_ws.cursor, sizeof(unsigned long long) * 8,
+                          hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    const uint64_t n = g_ws.h_res[4];
+    if (n == 0) break;
+    if (n > rcap) {
+      set_error("re-split: more pieces than the piece buffer holds");
+      return PM_ERR_OVERFLOW;
+    }
+    L.rin = (const Resume*)g_ws.resume[b];
+    L.nrin = n;
+    L.nrin_dev = nullptr;
+    L.rout = (Resume*)g_ws.resume[1 - b];
+    L.claim = 1;  // few, heavy pieces: one per wave
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, g_ws.cursor + 3, 2, 0ull);
+    hipLaunchKernelGGL((k_resume<false, true>), dim3(waves), dim3(kWave), 0, st, L);
+    HIP_OK(hipGetLastError());
+    b = 1 - b;
+    if (++*rounds > 100000) {
+      set_error("re-split: no progress");
+      return PM_ERR_DEVICE;
+    }
+  }
+  L.rin = nullptr;
+  L.nrin = 0;
+  L.rout = (Resume*)g_ws.resume[1];
+  L.claim = 0;
+  return PM_OK;
+}
+
 int device_match(pm_graph* g, const Plan& plan, const pm_opts* opts, pm_result* res) {
   const Plan* one = &plan;
   return device_match_batch(g, &one, 1, opts, res);
@@ -2727,7 +2834,20 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   const uint64_t stride = (uint64_t)std::max(nreg, 1) * cap;
   rc = grow(&g_ws.arena, &g_ws.arena_bytes, (size_t)waves * stride * sizeof(uint32_t));
   if (rc) return rc;
-  const uint64_t rcap = paged ? (1ull << 22) : 1;  // deferred pieces per round
+  // resident re-split budget (µs -> wall-clock ticks)
+  const int resplit_us = opts->resplit_us > 0 ? opts->resplit_us : 0;
+  // a search can re-split only above the fused level (depths <= k-4), so
+  // batches whose cores all have <= 4 vertices run without it
+  bool can_resplit = false;
+  for (int q = 0; q < npat; q++) can_resplit |= plans[q]->kc >= 5;
+  const uint64_t budget =
+      (paged || !can_resplit) ? 0
+                              : (uint64_t)resplit_us * (uint64_t)std::max(g_ws.wall_khz, 1) / 1000u;
+  const bool rs = budget != 0;
+  // deferred pieces per round (out of core) or re-split pieces per round
+  const uint64_t rcap = paged ? (1ull << 22)
+                        : budget ? std::max<uint64_t>(1ull << 20, (uint64_t)waves * 128)
+                                 : 1;
   for (int i = 0; i < 2; i++) {
     rc = grow(&g_ws.resume[i], &g_ws.resume_bytes[i], sizeof(Resume) * rcap);
     if (rc) return rc;
@@ -2817,6 +2937,7 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   L.task_cap = g_ws.task_cap;
   L.rout_cap = rcap;
   L.soft = rcap / 2;
+  L.budget = budget;
   const int reps = opts->repeats > 0 ? opts->repeats : 1;
   double sum_ms = 0, min_ms = 1e30, phase[3] = {0, 0, 0}, link_ms = 0, round_kernel_ms = 0;
   uint64_t link_bytes = 0, misses = 0, deferred = 0;
@@ -2883,8 +3004,11 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
       HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, g_ws.cursor, 16 + 2ull * npat, 0ull);
       launch_folds();
-      hipLaunchKernelGGL(k_match_edges<false>, dim3(waves), dim3(kWave), 0, st, L);
-      if (L.tasks) hipLaunchKernelGGL(k_match_tasks<false>, dim3(waves), dim3(kWave), 0, st, L);
+      hipLaunchKernelGGL((rs ? k_match_edges<false, true> : k_match_edges<false, false>),
+                         dim3(waves), dim3(kWave), 0, st, L);
+      if (L.tasks)
+        hipLaunchKernelGGL((rs ? k_match_tasks<false, true> : k_match_tasks<false, false>),
+                           dim3(waves), dim3(kWave), 0, st, L);
       const hipError_t le = hipGetLastError();
       hipGraph_t graph = nullptr;
       const hipError_t ce = hipStreamEndCapture(st, &graph);
@@ -2907,6 +3031,11 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   for (int r = 0; r < reps && use_graph; r++) {
     HIP_OK(hipEventRecord(g_ws.ev0, st));
     HIP_OK(hipGraphLaunch(g_ws.gexec, st));
+    rounds = 1;
+    if (budget) {
+      rc = resident_rounds(L, st, waves, rcap, &rounds);
+      if (rc) return rc;
+    }
     HIP_OK(hipEventRecord(g_ws.ev1, st));
     HIP_OK(hipEventSynchronize(g_ws.ev1));
     float ms = 0;
@@ -3008,10 +3137,21 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     launch_folds();
     HIP_OK(hipEventRecord(g_ws.evf, st));
     L.rout = (Resume*)g_ws.resume[1];
-    hipLaunchKernelGGL(paged ? k_match_edges<true> : k_match_edges<false>, dim3(waves), dim3(kWave), 0, st, L);
+    hipLaunchKernelGGL((paged ? k_match_edges<true, false>
+                        : rs  ? k_match_edges<false, true>
+                              : k_match_edges<false, false>),
+                       dim3(waves), dim3(kWave), 0, st, L);
     HIP_OK(hipEventRecord(g_ws.eve, st));
-    if (L.tasks) hipLaunchKernelGGL(paged ? k_match_tasks<true> : k_match_tasks<false>, dim3(waves), dim3(kWave), 0, st, L);
+    if (L.tasks)
+      hipLaunchKernelGGL((paged ? k_match_tasks<true, false>
+                          : rs  ? k_match_tasks<false, true>
+                                : k_match_tasks<false, false>),
+                         dim3(waves), dim3(kWave), 0, st, L);
     HIP_OK(hipGetLastError());
+    if (budget) {
+      rc = resident_rounds(L, st, waves, rcap, &rounds);
+      if (rc) return rc;
+    }
     HIP_OK(hipEventRecord(g_ws.ev1, st));
     HIP_OK(hipEventSynchronize(g_ws.ev1));
     float ms = 0, mf = 0, me = 0, mt = 0;
@@ -3023,15 +3163,15 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
     phase[1] = me;
     phase[2] = mt;
     if (paged) {
-      RoundStats rs;
-      rc = run_rounds(pg, L, st, waves, (uint64_t)npat * nchunk, rcap, opts, by_score, use_acc, rs);
+      RoundStats rst;
+      rc = run_rounds(pg, L, st, waves, (uint64_t)npat * nchunk, rcap, opts, by_score, use_acc, rst);
       if (rc) return rc;
-      misses = rs.misses;
-      deferred = rs.deferred;
-      link_bytes += rs.link_bytes;
-      link_ms += rs.link_ms;
-      round_kernel_ms += rs.kernel_ms;
-      rounds = rs.rounds;
+      misses = rst.misses;
+      deferred = rst.deferred;
+      link_bytes += rst.link_bytes;
+      link_ms += rst.link_ms;
+      round_kernel_ms += rst.kernel_ms;
+      rounds = rst.rounds;
     }
     const double wall =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -49,6 +49,8 @@ def test_random_options_match_oracle(seed):
             kw["order"] = "file"
         if rng.random() < 0.4:
             kw["graph"] = True
+        if rng.random() < 0.5:
+            kw["resplit_us"] = int(rng.choice([-1, 1, 5, 100]))
         print(f"[stress] seed={seed} trial={trial} {kw}", flush=True)
         got = [r.count for r in pm.match_batch(g, pats, **kw)]
         assert got == want, kw
