@@ -215,6 +215,7 @@ struct WaveState {
   unsigned long long acc_count, acc_jobs;  // phase 1: totals of pattern `pat`
   unsigned long long t0;  // wall clock at the start of the current search
   int32_t rs_depth, rs_stop;  // pending re-split (depth, first open depth); -1: none
+  int32_t rs_from;            // its first candidate at rs_depth (-1: after cidx)
   uint32_t blk;     // package of the last row found missing
   int32_t rdepth;   // resume input: depth, candidate indices, range end
   uint32_t ridx[kMaxDepth];
@@ -750,6 +751,13 @@ __device__ void emit_one(const DeferOut& O, const WaveState& ws, int depth, uint
   }
 }
 
+// the re-split budget of the current search has run out and the piece
+// buffer has room (uniform)
+__device__ __forceinline__ bool resplit_due(const DeferOut& O, const WaveState& ws) {
+  return O.budget && uni((uint32_t)(wall_clock64() - uni64(ws.t0) > O.budget)) &&
+         uni64(*(volatile unsigned long long*)O.count) < O.soft;
+}
+
 // A resident search stopped by its re-split budget (run_job recorded
 // rs_depth / rs_stop, right after the subtree of candidate cidx[rs_depth]
 // was finished): the rest of that depth goes out as up to 64 pieces, the
@@ -760,7 +768,8 @@ __device__ __forceinline__ void resplit_flush(const DeferOut& O, WaveState& ws, 
   if (PG || !RS) return;
   const int dd = (int)uni((uint32_t)ws.rs_depth);
   if (dd < 0) return;
-  const uint32_t a0 = uni(ws.cidx[dd]) + 1, b0 = uni(ws.clen[dd]);
+  const int from = (int)uni((uint32_t)ws.rs_from);
+  const uint32_t a0 = from >= 0 ? (uint32_t)from : uni(ws.cidx[dd]) + 1, b0 = uni(ws.clen[dd]);
   if (a0 < b0) {
     const uint32_t np = min(b0 - a0, 64u), per = (b0 - a0 + np - 1) / np;
     for (uint32_t a = a0; a < b0; a += per) emit_one(O, ws, dd, a, min(a + per, b0), 0, lane);
@@ -804,7 +813,7 @@ __device__ void emit_lanes(const DeferOut& O, const WaveState& ws, int depth, bo
 // [cnext[d], clen[d]).  Batches of 64 candidates are all-or-nothing: when a
 // needed row is not resident, *fail = the batch's first index and the sum of
 // the finished batches is returned.  Uniform result.
-template <bool PG>
+template <bool PG, bool RS = false>
 __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveState& ws, int d,
                                          uint32_t lane, Counters& ct, const DeferOut& O,
                                          bool* stopped) {
@@ -869,6 +878,15 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
   if (P.last_kind == 1 && !lab) {
     // u2's set is N(x): per-lane binary searches
     for (uint32_t base = cbeg; base < cend; base += kWave) {
+      if (RS && !PG && base > cbeg && !listed && resplit_due(O, ws)) {
+        // re-split: the candidates from this batch on become pieces
+        if (lane == 0) {
+          ws.rs_depth = d;
+          ws.rs_from = (int32_t)base;
+        }
+        *stopped = true;
+        break;
+      }
       const uint32_t i = base + lane;
       const uint32_t* nx = nullptr;
       uint32_t dx = 0, x = 0, mp = 0;
@@ -950,6 +968,15 @@ __device__ unsigned long long fused_tail(const KGraph& g, const KPlan& P, WaveSt
   const bool pair = P.pair;
   const uint64_t hub_lim = isect ? (uint64_t)P.hub_factor * (uint64_t)max(nP, 1u) : ~0ull;
   for (uint32_t base = cbeg; base < cend; base += kWave) {
+    if (RS && !PG && base > cbeg && !listed && resplit_due(O, ws)) {
+      // re-split: the candidates from this batch on become pieces
+      if (lane == 0) {
+        ws.rs_depth = d;
+        ws.rs_from = (int32_t)base;
+      }
+      *stopped = true;
+      break;
+    }
     const uint32_t i = base + lane;
     uint32_t x = 0, dx = 0, ub = ub_fixed;
     unsigned long long wx = 0, wb = 0;
@@ -1295,6 +1322,7 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
     if (lane == 0) {
       ws.rs_depth = dd;
       ws.rs_stop = stop;
+      ws.rs_from = -1;
     }
     return true;
   };
@@ -1302,8 +1330,14 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
     if (fused_now) {
       fused_now = false;
       bool stopped = false;
-      cnt += fused_tail<PG>(g, P, ws, d, lane, ct, O, &stopped);
+      cnt += fused_tail<PG, RS>(g, P, ws, d, lane, ct, O, &stopped);
       if (stopped) {
+        if (RS && (int)uni((uint32_t)ws.rs_depth) >= 0) {
+          // re-split inside the fused step: resplit_flush writes its rest
+          // and the rest of the shallower depths
+          if (lane == 0) ws.rs_stop = stop;
+          return cnt;
+        }
         emit_rest(d);
         return cnt;
       }
@@ -2836,10 +2870,10 @@ int device_match_batch(pm_graph* g, const Plan* const* plans, int npat, const pm
   if (rc) return rc;
   // resident re-split budget (µs -> wall-clock ticks)
   const int resplit_us = opts->resplit_us > 0 ? opts->resplit_us : 0;
-  // a search can re-split only above the fused level (depths <= k-4), so
-  // batches whose cores all have <= 4 vertices run without it
+  // a search can re-split at depths 1..k-3 (the fused step included), so
+  // batches whose cores all have <= 3 vertices run without it
   bool can_resplit = false;
-  for (int q = 0; q < npat; q++) can_resplit |= plans[q]->kc >= 5;
+  for (int q = 0; q < npat; q++) can_resplit |= plans[q]->kc >= 4;
   const uint64_t budget =
       (paged || !can_resplit) ? 0
                               : (uint64_t)resplit_us * (uint64_t)std::max(g_ws.wall_khz, 1) / 1000u;

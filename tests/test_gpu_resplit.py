@@ -8,7 +8,7 @@ import pytest
 import oracle
 import pumatch_amd as pm
 from pumatch_amd.workloads import batch_patterns
-from tests.helpers import C7_EDGES, HT_EDGES, K5P_EDGES, edges_to_adj, g_text
+from tests.helpers import C1_EDGES, C7_EDGES, HT_EDGES, K5P_EDGES, edges_to_adj, g_text
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +39,9 @@ def oracle_count(g, text, labeled):
 
 
 CASES = [("c7", C7_EDGES, False), ("k5p", K5P_EDGES, False), ("ht", HT_EDGES, True),
-         ("ht", HT_EDGES, False), ("c5", [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], False)]
+         ("ht", HT_EDGES, False), ("c5", [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], False),
+         ("c1", C1_EDGES, True),  # K4 core: splits inside the fused step only
+         ("c4", [(0, 1), (1, 2), (2, 3), (3, 0)], False)]
 
 
 @pytest.mark.parametrize("budget", [1, 20, 0, -1])
