@@ -86,6 +86,11 @@ constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
 constexpr int kWalk = PM_FUSED_WALK;
 constexpr uint32_t kResumeClaim = 16;  // deferred pieces claimed per atomic
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+// heavy-root split cost of a vertex-2 candidate: 1 + its list length, or
+// for cores of 5+ vertices 1 + length^2 / 64 (build-time switch)
+#ifndef PM_SPLIT_QUAD
+#define PM_SPLIT_QUAD 1
+#endif
 // range-task buffer: 2^PM_TASK_CAP_LOG2 tasks of 32 B (build-time tunable)
 #ifndef PM_TASK_CAP_LOG2
 #define PM_TASK_CAP_LOG2 22
@@ -1204,7 +1209,17 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
       uint64_t run = 0, prev_c = 0, rank0 = 0;
       for (uint32_t base = b; base < en; base += kWave) {
         const uint32_t i = base + lane;
+#if PM_SPLIT_QUAD
+        // deeper cores: a candidate's subtree grows at least with the square
+        // of its list (scaled so a 64-entry list costs 64)
+        uint64_t cost = 0;
+        if (i < en) {
+          const uint64_t ln = row_len(g, cand[i], cseg);
+          cost = 1ull + (k >= 5 ? ln * ln / 64 : ln);
+        }
+#else
         const uint64_t cost = i < en ? 1ull + row_len(g, cand[i], cseg) : 0;
+#endif
         uint64_t incl = cost;
         for (int o = 1; o < kWave; o <<= 1) {
           const uint64_t t = __shfl_up(incl, o);
