@@ -86,10 +86,11 @@ constexpr int kHashCap = 1 << kHashBits;  // LDS hash of the fused parent set
 constexpr int kWalk = PM_FUSED_WALK;
 constexpr uint32_t kResumeClaim = 16;  // deferred pieces claimed per atomic
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-// heavy-root split cost of a vertex-2 candidate: 1 + its list length, or
-// for cores of 5+ vertices 1 + length^2 / 64 (build-time switch)
+// heavy-root split cost of a vertex-2 candidate (build-time switch):
+// 0: 1 + its list length; 1: 1 + length^2 / 64 for cores of 5+ vertices;
+// 2: 1 + length^(k-3) / 64^(k-4) for cores of k >= 4 vertices
 #ifndef PM_SPLIT_QUAD
-#define PM_SPLIT_QUAD 1
+#define PM_SPLIT_QUAD 2
 #endif
 // range-task buffer: 2^PM_TASK_CAP_LOG2 tasks of 32 B (build-time tunable)
 #ifndef PM_TASK_CAP_LOG2
@@ -1210,12 +1211,21 @@ __device__ unsigned long long run_job(const KGraph& g, const KPlan& P, WaveState
       for (uint32_t base = b; base < en; base += kWave) {
         const uint32_t i = base + lane;
 #if PM_SPLIT_QUAD
-        // deeper cores: a candidate's subtree grows at least with the square
-        // of its list (scaled so a 64-entry list costs 64)
+        // deeper cores: a candidate's subtree grows with a power of its list
+        // (scaled so that a 64-entry list costs 64 at every depth)
         uint64_t cost = 0;
         if (i < en) {
           const uint64_t ln = row_len(g, cand[i], cseg);
-          cost = 1ull + (k >= 5 ? ln * ln / 64 : ln);
+          uint64_t cw = ln;
+#if PM_SPLIT_QUAD >= 2
+          const int extra = k - 4;  // len^(k-3) / 64^(k-4)
+#else
+          const int extra = k >= 5 ? 1 : 0;  // len^2 / 64
+#endif
+          // capped at 2^40 per candidate (no overflow: cw * ln <= 2^46 here)
+          for (int q = 0; q < extra; q++)
+            cw = (ln && cw > (1ull << 46) / ln) ? (1ull << 40) : min(cw * ln / 64, 1ull << 40);
+          cost = 1ull + cw;
         }
 #else
         const uint64_t cost = i < en ? 1ull + row_len(g, cand[i], cseg) : 0;
