@@ -116,14 +116,16 @@ typedef struct {
   int32_t waves;        /* wavefronts actually launched */
   int32_t chunk;
   /* per kernel phase of the last timed launch (hipEvents on the launch
-     stream): 0 pendant-tree folding, 1 edge phase, 2 range-task phase */
+     stream): 0 pendant-tree folding, 1 edge phase, 2 range-task phase
+     (with re-split rounds, if any); all 0 in graph mode */
   double phase_ms[3];
   uint64_t phase_alg_bytes[3];
   uint64_t misses;      /* out-of-core: package misses over all rounds */
   uint64_t deferred;    /* out-of-core: branches deferred and resumed */
-  int32_t rounds;       /* out-of-core: matching rounds (1 when resident) */
+  int32_t rounds;       /* out-of-core matching rounds, or resident re-split
+                           rounds + 1 (1 when neither ran) */
   int32_t packages;     /* out-of-core: packages in the graph */
-  double link_ms;       /* out-of-core: host->HBM copy time */
+  double link_ms;       /* out-of-core: host->HBM copy time (preload included) */
   double round_kernel_ms; /* out-of-core: kernel time of rounds 2.. (the rest of
                            kernel_ms minus link_ms is host-side round work) */
 } pm_result;
