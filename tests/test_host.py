@@ -220,3 +220,48 @@ def test_batch_and_tune_argument_errors():
     assert L.pm_model_get(None, None) == pm.PM_ERR_ARG
     assert L.pm_model_predict(None, None, 0, None) == pm.PM_ERR_ARG
     assert L.pm_model_train(None, None, None, None, 1, None, None) == pm.PM_ERR_ARG
+
+
+def _header_fields(struct_name):
+    """(name, C type, array length) of each field of `typedef struct {...}
+    struct_name;` in include/pumatch.h, in order."""
+    import re
+    text = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                             "include", "pumatch.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    m = re.search(r"typedef struct \{([^{}]*)\}\s*" + struct_name + r";", text)
+    assert m, struct_name
+    out = []
+    for decl in m.group(1).split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        dm = re.match(r"(const )?(struct )?([A-Za-z_0-9]+)\s*(\*?)\s*(.*)", decl)
+        ctype, ptr, names = dm.group(3), dm.group(4), dm.group(5)
+        for nm in names.split(","):
+            nm = nm.strip()
+            star = ptr or nm.startswith("*")
+            nm = nm.lstrip("*").strip()
+            am = re.match(r"([A-Za-z_0-9]+)(\[([A-Z_0-9]+)\])?", nm)
+            out.append((am.group(1), "ptr" if star else ctype, am.group(3)))
+    return out
+
+
+@pytest.mark.parametrize("struct_name,cls", [("pm_opts", "Opts"), ("pm_result", "Result"),
+                                             ("pm_pattern_info", "PatternInfo"),
+                                             ("pm_graph_info", "GraphInfo"),
+                                             ("pm_train_opts", "TrainOpts"),
+                                             ("pm_tune_opts", "TuneOpts"),
+                                             ("pm_tune_report", "TuneReport")])
+def test_ctypes_mirrors_match_the_header(struct_name, cls):
+    import ctypes as C
+    widths = {"int32_t": 4, "uint32_t": 4, "int64_t": 8, "uint64_t": 8, "double": 8,
+              "float": 4, "ptr": 8}
+    fields = _header_fields(struct_name)
+    mirror = getattr(pm, cls)._fields_
+    assert [f[0] for f in fields] == [f[0] for f in mirror], struct_name
+    for (name, ctype, arr), (_, mtype) in zip(fields, mirror):
+        n = 1
+        if arr:
+            n = {"PM_MAX_PATTERN": 7}.get(arr, None) or int(arr)
+        assert C.sizeof(mtype) == widths[ctype] * n, (struct_name, name)
