@@ -161,3 +161,33 @@ def test_per_edge_induced_counts_sum_to_total():
     total, _, pe = run(rowptr, colidx, None, 5, adj, False, per_edge=True)
     aut = oracle.plan_info(oracle.parse_g(g_text(5, adj), False), False)["aut"]
     assert int(pe.sum()) * aut == total
+
+
+def test_oracle_cli_modes_match_the_library(tmp_path):
+    """pm_oracle <prefix> <pattern.g> [--unlabeled] [--vertex-induced] prints
+    the same totals as the oracle library for the four modes."""
+    import subprocess
+
+    import pumatch_amd as pm
+    g = pm.Graph.rmat(8, 8, seed=12, device=False)
+    g.set_uniform_labels(2, 3)
+    prefix = str(tmp_path / "g")
+    g.write(prefix)
+    text = g_text(5, edges_to_adj(5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (0, 2)]),
+                  [0, 1, 0, 1, 1])
+    pat = tmp_path / "house.g"
+    pat.write_text(text)
+    rp, ci, lb = g.csr()
+    for labeled in (True, False):
+        for induced in (False, True):
+            args = [oracle.CLI_PATH, prefix, str(pat)]
+            if not labeled:
+                args.append("--unlabeled")
+            if induced:
+                args.append("--vertex-induced")
+            out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+            assert out.returncode == 0, out.stderr
+            got = int(out.stdout.strip().splitlines()[-1].split()[1])
+            want, _, _ = oracle.count(rp, ci, lb, oracle.parse_g(text, labeled), labeled,
+                                      induced=induced)
+            assert got == want, (labeled, induced)
